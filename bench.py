@@ -1,0 +1,154 @@
+"""Benchmark: FP64 ADI time step of the synthetic meandering river
+(980 x 1048 cells, dx = dy = 20 m, dT = 5 s, hydrodynamics only) on one
+MI355X, in wet-cell-steps per second.
+
+A step is one full ADI step (UZ + VZ half-steps) of the whole river; the
+timed region replays K captured steps (hd_run) with the state resident in
+HBM.  One JSON line is printed with the throughput, the roofline of the
+dominant kernel (per-launch HIP-event durations recorded on the library's
+stream inside this process) and the CPU checker timed on a bounded sample on
+this host's cores.
+
+    python bench.py [--gpus 1] [--steps 720] [--warmup 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-hydist_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+B_STEP = 256               # algorithmic bytes per wet-cell-step (hydrodynamics)
+
+# Algorithmic bytes per wet cell of the per-launch kernels (compulsory FP64
+# reads/writes of the kernel's inputs and outputs, each element counted once).
+KERNEL_BYTES = {
+    # rows A,B,C,D of two rows read, solution (z and u) written
+    "adi_solve_uz": 2 * 4 * 8 + 2 * 8,
+    "adi_solve_vz": 2 * 4 * 8 + 2 * 8,
+    # u,v,z,Htdu,Htdv,H_moi,Kx1|Ky1,VISCOIDX,Tsxw|Tsyw read, two rows written
+    "adi_coef_uz": 9 * 8 + 2 * 4 * 8,
+    "adi_coef_vz": 9 * 8 + 2 * 4 * 8,
+}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=720)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--profile-steps", type=int, default=20)
+    p.add_argument("--cpu-steps", type=int, default=100)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(case, nsteps):
+    """The CPU checker (same FP64 method, one thread) on the first `nsteps`
+    steps of the same run."""
+    import oracle
+    o = oracle.Oracle(case)
+    o.init()
+    opts = case.opts()
+    wet = 0
+    t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        wet += o.wet_count()
+        t = o.step(t, opts)
+    dt = time.perf_counter() - t0
+    o.close()
+    return wet / dt, dt
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    from hydist_amd import synth
+    from hydist_amd._lib import Solver
+
+    case = synth.config(2)
+    opts = case.opts()
+    g = Solver(case, device=0)
+    g.init()
+    t = g.run(0.0, a.warmup, opts) if a.warmup else 0.0
+    g.sync()
+    wc0 = g.wet_count()
+    g.reset_wet_counter()
+    g.sync()
+    t0 = time.perf_counter()
+    t = g.run(t, a.steps, opts, counted=True)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    wc1 = g.wet_count()
+    wet_steps = g.wet_cell_steps() + wc0 - wc1   # start states of the K timed steps
+    value = wet_steps / elapsed
+    ms_per_step = elapsed * 1e3 / a.steps
+
+    # per-kernel launch durations (HIP events on the library's stream)
+    g.profile(True)
+    g.run(t, a.profile_steps, opts)
+    kt = g.kernel_times()
+    g.profile(False)
+    wet_mean = wet_steps / a.steps
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_n) = dom
+    avg_s = dom_ms / dom_n * 1e-3
+    per_cell = KERNEL_BYTES.get(dom_name)
+    roof = {"kernel": dom_name, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK / 1e9,
+            "avg_launch_us": avg_s * 1e6, "launches": dom_n}
+    if per_cell is not None:
+        bytes_launch = per_cell * wet_mean
+        roof.update(achieved=bytes_launch / avg_s / 1e9, frac=bytes_launch / avg_s / HBM_PEAK,
+                    bytes_per_wet_cell=per_cell, algorithmic_bytes_per_launch=bytes_launch)
+    else:
+        roof.update(achieved=None, frac=None)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom_name)
+    roof["traffic"] = traffic
+    kernels = {k: round(ms / n * 1e3, 2) for k, (ms, n) in kt.items()}
+
+    cpu = None
+    if not a.no_cpu:
+        rate, dt = cpu_baseline(case, a.cpu_steps)
+        cpu = {"value": rate, "unit": "wet-cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"first {a.cpu_steps} steps of river_980x1048 from t=0, CPU checker, 1 thread "
+                         f"({dt:.1f} s)"}
+
+    line = {
+        "metric": "ADI wet-cell-steps/s (FP64)",
+        "value": value,
+        "unit": "wet-cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": case.name, "N": case.N, "M": case.M, "dx_m": 20.0, "dT_s": case.dT,
+                   "mean_wet_cells": round(wet_mean, 1), "physics": "hydrodynamics",
+                   "hbm_frac_step_algorithmic": value * B_STEP / HBM_PEAK},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernel_us": kernels,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+/*
+ * hd_internal.h -- device data layout shared by the libhydist kernels.
+ *
+ * HBM layout (one context = one case on one GPU):
+ *  - every 2-D field of the reference (Argument_Pointers, constant.cuh:6-20)
+ *    is its own (N+3) x (M+3) row-major FP64 array, flat index i*(M+3)+j, so
+ *    uploads and downloads are the caller's numpy arrays byte for byte;
+ *  - t_u and t_v each have a second buffer: the smoothing pass
+ *    (Normalize/update_buffer, support_funcs.cu:682-729) writes the other
+ *    buffer and the two swap roles, instead of a copy-back kernel;
+ *  - the tridiagonal rows of a half-step are stored line-major like the
+ *    reference's AA/BB/CC/DD (UVZSolver_multithread.cu:489-492): line L, row r
+ *    at L * tw + r, tw = 2*len+2, with the elimination factors Ap/Bp beside
+ *    them; per-segment continuity coefficients at the two line ends live in
+ *    a small side table (SEG_*).
+ * Kernels receive HdDev by value (kernel arguments), so a captured HIP graph
+ * replays with exactly the pointers of the step it recorded.
+ */
+#ifndef HD_INTERNAL_H
+#define HD_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include "../../include/hydist.h"
+#include "hd_math.h"
+
+struct HdDev {
+    int N, M, W, SL, total_time, series_len;
+    hd_params P;
+    /* fields */
+    double *h, *hsnham, *VISCOIDX, *u, *v, *z, *t_u, *t_v, *t_z, *Htdu, *Htdv, *H_moi, *htaiz, *htaiz_bd;
+    double *Kx1, *Ky1, *Tsxw, *Tsyw, *FS, *tFS, *VTH, *Kx, *Ky, *Fw, *Qbx, *Qby, *dH;
+    int* khouot;
+    double *ubt, *ubp, *vbt, *vbd, *hi;
+    double *bc_up, *bc_down, *bc_left, *bc_right, *CC_u, *CC_d, *CC_l, *CC_r;
+    int *bienQ, *boundary_type, *daui, *cuoii, *dauj, *cuoij, *moci, *mocj;
+    double* hmax; /* [4]: hmax_u, hmax_d, hmax_l, hmax_r */
+    /* scratch */
+    double *tmp_u, *tmp_v;          /* smoothing ping-pong partners of t_u / t_v */
+    double *AA, *BB, *CC, *DD, *Ap, *Bp;
+    double* seg_end;                /* per line*SL+seg: a2f,c2f,d2f,a2l,c2l,d2l */
+    int* seg_sn;                    /* per line*SL+seg: system size sn */
+    int* flags;                     /* [0]: segment overflow, [1]: boundary range */
+    long long* wet_count;
+};
+
+/* kernel launchers (hd_kernels.hip, hd_sediment.hip), one kernel each */
+void hd_launch_onetime_init(const HdDev& d, hipStream_t s);
+void hd_launch_find_limits(const HdDev& d, hipStream_t s); /* segments + fs_edges */
+void hd_launch_segments(const HdDev& d, int count, hipStream_t s);
+void hd_launch_fs_edges(const HdDev& d, hipStream_t s);
+void hd_launch_htuongdoi(const HdDev& d, hipStream_t s);
+void hd_launch_preprocess(const HdDev& d, hipStream_t s);
+void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s);
+void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
+                              unsigned char* dryflag, hipStream_t s);
+void hd_launch_explicit(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_margins(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_wetting(const HdDev& d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag,
+                       hipStream_t s);
+void hd_launch_uvz_dry(const HdDev& d, int kd, int kn, const unsigned char* dryflag, hipStream_t s);
+void hd_launch_vth_hesok(const HdDev& d, hipStream_t s);
+void hd_launch_fs_coef(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s);
+void hd_launch_fs_solve(const HdDev& d, int which, hipStream_t s);
+void hd_launch_update_fs(const HdDev& d, hipStream_t s);
+void hd_launch_calc_qb(const HdDev& d, int qb_mode, hipStream_t s);
+void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStream_t s);
+void hd_launch_wet_count(const HdDev& d, hipStream_t s);
+int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
+                           const double* C, const double* D, double* x, double* Ap, double* Bp, hipStream_t s);
+
+#endif

@@ -1,0 +1,1207 @@
+/*
+ * hd_kernels.hip -- gfx950 kernels of the FP64 ADI step.
+ *
+ * One half-step (UZ shown; VZ is the transpose) is
+ *   boundary -> coef (one thread per wet cell writes its two tridiagonal rows)
+ *   -> solve (one lane per line: Thomas over each wet run, solution scattered
+ *      straight into t_z / t_u during back substitution)
+ *   -> smooth t_u -> explicit v -> v margins -> smooth t_v (+ H_moi, drying
+ *      flags, wetting candidates) -> wetting (serial only at candidates)
+ *   -> state copy (+ drying, face depths) -> segment tables (+ FS edge
+ *      interpolation).
+ * Every formula follows the reference kernel it replaces (cited per function)
+ * in the same IEEE operation order; the code is built with -ffp-contract=off
+ * so the bits match the CPU checker.  Wave size is 64.
+ */
+#include "hd_internal.h"
+
+#define W_ (d.W)
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kCand = 64;   /* wetting candidates kept per line */
+
+__device__ __forceinline__ unsigned long long lanemask_lt()
+{
+    unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+/* ================================================================ init */
+
+/* Onetime_init, support_funcs.cu:20-58: every cell ends as khouot = 2 except
+ * the cells of [1,N+1]x[1,M+1] whose 4-node mean depth exceeds NANGDAY. */
+__global__ void k_onetime_init(HdDev d)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    int N = d.N, M = d.M, W = d.W;
+    if (i >= N + 3 || j >= M + 3) return;
+    const double* h = d.h;
+    int p = i * W + j;
+    int kh = 2;
+    if (i >= 1 && i <= N + 1 && j >= 1 && j <= M + 1) {
+        double s4 = (h[(i - 1) * W + j - 1] + h[(i - 1) * W + j] + h[i * W + j - 1] + h[i * W + j]) * 0.25;
+        if (s4 > d.P.NANGDAY) {
+            kh = 0;
+            d.H_moi[p] = 0;
+        }
+        if (i <= N && j <= M) {
+            double ht = (h[(i - 1) * W + j - 1] + h[(i - 1) * W + j] + h[i * W + j - 1] + h[i * W + j]) * 0.25;
+            d.htaiz[p] = ht;
+            d.htaiz_bd[p] = ht;
+            if (h[i * W + j - 1] + h[p] != 0)
+                d.Kx1[p] = d.P.g * hd_pow((h[i * W + j - 1] + h[p]) * 0.5, -2 * d.P.mu_mn)
+                           * hd_pow((d.hsnham[i * W + j - 1] + d.hsnham[p]) * 0.5, 2);
+            if (h[(i - 1) * W + j] + h[p] != 0)
+                d.Ky1[p] = d.P.g * hd_pow((h[(i - 1) * W + j] + h[p]) * 0.5, -2 * d.P.mu_mn)
+                           * hd_pow((d.hsnham[(i - 1) * W + j] + d.hsnham[p]) * 0.5, 2);
+        }
+    }
+    d.khouot[p] = kh;
+}
+
+/* Htuongdoi, support_funcs.cu:338-354 (init form, from z). */
+__global__ void k_htuongdoi(HdDev d)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int i = blockIdx.y + 1;
+    int W = d.W;
+    if (i > d.N || j > d.M) return;
+    const double *h = d.h, *z = d.z;
+    d.Htdu[i * W + j] = (h[i * W + j - 1] + h[i * W + j] + z[(i + 1) * W + j] + z[i * W + j]) * 0.5;
+    d.Htdv[i * W + j] = (h[(i - 1) * W + j] + h[i * W + j] + z[i * W + j + 1] + z[i * W + j]) * 0.5;
+}
+
+/* One edge of preprocess_data, support_funcs.cu:450-511: hi = hbar^(2/3) /
+ * (sum hbar^(5/3) dX) per wet run of the edge line, summed serially in index
+ * order (the reference runs it single-threaded). */
+__device__ void preprocess_edge(const HdDev& d, int edge)
+{
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const double* h = d.h;
+    double* hi = d.hi;
+    const int *moc, *dau, *cuoi;
+    int line;
+    if (edge == 0) { moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; line = M; }
+    else if (edge == 1) { moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; line = 2; hi += N + 3; }
+    else if (edge == 2) { moc = d.moci; dau = d.daui; cuoi = d.cuoii; line = 2; hi += 2 * (N + 3); }
+    else { moc = d.moci; dau = d.daui; cuoi = d.cuoii; line = N; hi += 2 * (N + 3) + (M + 3); }
+    for (int k = 0; k < moc[line]; ++k) {
+        double sum = 0;
+        int a = dau[line * SL + k], b = cuoi[line * SL + k];
+        for (int i = a; i <= b; ++i) {
+            double v;
+            if (edge == 0) v = (h[i * W + M] + h[(i - 1) * W + M]) / 2.0 - d.P.NANGDAY;
+            else if (edge == 1) v = (h[i * W + 1] + h[(i - 1) * W + 1]) / 2.0 - d.P.NANGDAY;
+            else if (edge == 2) v = (h[1 * W + i] + h[1 * W + i - 1]) / 2.0 - d.P.NANGDAY;
+            else v = (h[N * W + i] + h[N * W + i - 1]) / 2.0 - d.P.NANGDAY;
+            hi[i] = v;
+            sum += hd_pow(v, 5.0 / 3.0);
+        }
+        for (int i = a; i <= b; ++i) hi[i] = hd_pow(hi[i], 2.0 / 3.0) / (sum * d.P.dX);
+    }
+}
+
+/* preprocess_data, support_funcs.cu:433-542: the four edges (independent hi
+ * ranges) in parallel, then hmax_* = 1/(ros * max htaiz_bd) per edge. */
+__global__ void k_preprocess(HdDev d)
+{
+    int e = threadIdx.x;
+    if (e < 4) preprocess_edge(d, e);
+    if (e != 0) return;
+    int N = d.N, M = d.M, W = d.W;
+    const double* hb = d.htaiz_bd;
+    double hu = hb[2 * W + M], hd = hb[2 * W + 2];
+    for (int i = 3; i < N; ++i) {
+        if (hb[i * W + M] > hu) hu = hb[i * W + M];
+        if (hb[i * W + 2] > hd) hd = hb[i * W + 2];
+    }
+    double hl = hb[2 * W + 2], hr = hb[N * W + 2];
+    for (int i = 3; i < M; ++i) {
+        if (hb[2 * W + i] > hl) hl = hb[2 * W + i];
+        if (hb[N * W + i] > hr) hr = hb[N * W + i];
+    }
+    if (hu != 0) hu = 1.0 / (d.P.ros * hu);
+    if (hd != 0) hd = 1.0 / (d.P.ros * hd);
+    if (hl != 0) hl = 1.0 / (d.P.ros * hl);
+    if (hr != 0) hr = 1.0 / (d.P.ros * hr);
+    d.hmax[0] = hu; d.hmax[1] = hd; d.hmax[2] = hl; d.hmax[3] = hr;
+}
+
+/* ================================================================ boundary */
+
+/* Boundary_value + FS_boundary for thread index i of one edge,
+ * support_funcs.cu:544-617.  Only segment 0 of the edge line can match
+ * (the `break` at support_funcs.cu:558). */
+__device__ void edge_value(const HdDev& d, int edge, int i, int t1, double t2)
+{
+    int N = d.N, M = d.M, W = d.W, SL = d.SL, TT = d.total_time;
+    const int* bt = d.boundary_type + edge * SL;
+    const int *moc, *dau, *cuoi;
+    int location, loc_ext;
+    bool isU;
+    const double *bc, *CCs, *hi;
+    double* barr;
+    double hmax = d.hmax[edge];
+    if (edge == 0) { isU = false; location = M; loc_ext = M + 1; moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; bc = d.bc_up; CCs = d.CC_u; hi = d.hi; barr = d.vbt; }
+    else if (edge == 1) { isU = false; location = 2; loc_ext = 1; moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; bc = d.bc_down; CCs = d.CC_d; hi = d.hi + (N + 3); barr = d.vbd; }
+    else if (edge == 2) { isU = true; location = 2; loc_ext = 1; moc = d.moci; dau = d.daui; cuoi = d.cuoii; bc = d.bc_left; CCs = d.CC_l; hi = d.hi + 2 * (N + 3); barr = d.ubt; }
+    else { isU = true; location = N; loc_ext = N + 1; moc = d.moci; dau = d.daui; cuoi = d.cuoii; bc = d.bc_right; CCs = d.CC_r; hi = d.hi + 2 * (N + 3) + (M + 3); barr = d.ubp; }
+    if (!(moc[location] > 0 && dau[location * SL] <= i && i <= cuoi[location * SL])) return;
+    const int seg_no = 0;
+    double bv = bc[seg_no * TT + t1] * (1.0 - t2) + bc[seg_no * TT + t1 + 1] * t2;
+    if (bt[seg_no] == 2) {
+        barr[i] = bv * hi[i];
+    } else if (isU) {
+        d.t_z[location * W + i] = bv;
+        d.t_z[loc_ext * W + i] = bv;
+    } else {
+        d.t_z[i * W + location] = bv;
+        d.t_z[i * W + loc_ext] = bv;
+    }
+    if (bt[seg_no] != 2) return;
+    int fi, fj;
+    if (isU) { fi = location; fj = i; } else { fj = location; fi = i; }
+    double cv = CCs[seg_no * TT + t1] * (1.0 - t2) + CCs[seg_no * TT + t1 + 1] * t2;
+    d.FS[fi * W + fj] = cv * d.htaiz_bd[fi * W + fj] * hmax;
+}
+
+/* boundary_up/down/left/right, support_funcs.cu:356-429 (analytic tide of the
+ * narrow-channel mode), thread index i of one edge. */
+__device__ void edge_analytic(const HdDev& d, int edge, int i, double t)
+{
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const double PI = d.P.PI, dX = d.P.dX, dY = d.P.dY;
+    if (edge == 0) {
+        if (i < d.dauj[M * SL] || i > d.cuoij[M * SL]) return;
+        d.t_z[i * W + M] = 0.01 * hd_cos(2 * PI / 27.750 * t) * hd_cos(2 * (PI / 100) * (100 - dY / 2));
+        d.t_z[i * W + M + 1] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * (100 + dY / 2));
+    } else if (edge == 1) {
+        if (i < d.dauj[2 * SL] || i > d.cuoij[2 * SL]) return;
+        if (d.bienQ[1]) d.vbd[i] = 0;
+        else {
+            d.t_z[i * W + 2] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * dY / 2);
+            d.t_z[i * W + 1] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * (-dY) / 2);
+        }
+    } else if (edge == 2) {
+        if (i < d.daui[2 * SL] || i > d.cuoii[2 * SL]) return;
+        if (d.bienQ[2]) d.ubt[i] = 0;
+        else {
+            d.t_z[2 * W + i] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * dX / 2);
+            d.t_z[1 * W + i] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * (-dX) / 2);
+        }
+    } else {
+        if (i < d.daui[2 * SL] || i > d.cuoii[N * SL]) return;
+        if (d.bienQ[3]) d.ubp[i] = 0;
+        else {
+            d.t_z[N * W + i] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * (100 - dX / 2));
+            d.t_z[(N + 1) * W + i] = 0.01 * hd_cos(2 * PI / 27.75 * t) * hd_cos(2 * (PI / 100) * (100 + dX / 2));
+        }
+    }
+}
+
+/* Update_Boundary_Value, support_funcs.cu:622-668 (or the analytic channel
+ * boundaries), one workgroup; the edges run in the reference's order
+ * up, down, left, right, so a corner cell keeps the later edge's value.
+ * The half-step time t = t_prev + dT/2 is advanced here, on the device. */
+__global__ void k_boundary(HdDev d, double* tdev, int analytic)
+{
+    __shared__ double t_s;
+    if (threadIdx.x == 0) t_s = tdev[0] + d.P.dT * 0.5;
+    __syncthreads();
+    double t = t_s;
+    if (threadIdx.x == 0) tdev[0] = t;
+    int imax = d.M > d.N ? d.M : d.N;
+    if (analytic) {
+        int lo = 0, hi = imax + 2;
+        for (int e = 0; e < 4; ++e) {
+            for (int i = lo + threadIdx.x; i <= hi; i += blockDim.x) edge_analytic(d, e, i, t);
+            __syncthreads();
+        }
+        return;
+    }
+    int t1 = (int)(t / 3600);
+    double t2 = (t - (3600.0 * t1)) / 3600.0;
+    if (t1 < 0 || t1 + 1 >= d.total_time || t1 + 1 >= d.series_len) {
+        if (threadIdx.x == 0) d.flags[1] = 1;
+        return;
+    }
+    for (int e = 0; e < 4; ++e) {
+        for (int i = 2 + threadIdx.x; i <= imax; i += blockDim.x) edge_value(d, e, i, t1, t2);
+        __syncthreads();
+    }
+}
+
+/* ================================================================ ADI rows */
+
+struct Pre {
+    double f1, f2, f3, f5, a2, c2, d2;
+};
+
+/* _uzSolver_calculate_preindex, UVZSolver_multithread.cu:352-407, cell (i, j). */
+__device__ Pre pre_uz(const HdDev& d, int i, int j)
+{
+    const hd_params& P = d.P;
+    int W = d.W;
+    const double *u = d.u, *v = d.v, *Hm = d.H_moi, *VIS = d.VISCOIDX;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    double uc = u[i * W + j];
+    double vtb = (v[i * W + j - 1] + v[i * W + j] + v[(i + 1) * W + j - 1] + v[(i + 1) * W + j]) * 0.25;
+    double vis = VIS[i * W + j];
+    double htu = d.Htdu[i * W + j];
+    r.f1 = P.dTchia2dX * uc + vis * P.dT / P.dXbp;
+    r.f2 = -(2 + d.Kx1[i * W + j] * P.dT * sqrt(uc * uc + vtb * vtb) / htu + (2 * P.dT * vis) / P.dXbp);
+    r.f3 = P.dT * vis / P.dXbp - P.dTchia2dX * uc;
+    if (Hm[i * W + j - 1] <= P.H_TINH) {
+        if (vtb < 0) {
+            p = vtb * (-3 * uc + 4 * u[i * W + j + 1] - u[i * W + j + 2]) / P.dY2;
+            q = (uc - 2 * u[i * W + j + 1] + u[i * W + j + 2]) / P.dYbp;
+        }
+    } else {
+        if (Hm[i * W + j + 1] <= P.H_TINH) {
+            if ((Hm[i * W + j - 2] > P.H_TINH) && (vtb > 0)) {
+                p = vtb * (3 * uc - 4 * u[i * W + j - 1] + u[i * W + j - 2]) / P.dY2;
+                q = (uc - 2 * u[i * W + j - 1] + u[i * W + j - 2]) / P.dYbp;
+            }
+        } else {
+            p = vtb * (u[i * W + j + 1] - u[i * W + j - 1]) / P.dY2;
+            q = (u[i * W + j + 1] - 2 * uc + u[i * W + j - 1]) / P.dYbp;
+        }
+    }
+    r.f5 = -2 * uc + P.dT * p - P.dT * P.CORIOLIS_FORCE * vtb - P.dT * vis * q - P.dT * (P.Windx - d.Tsxw[i * W + j]) / htu;
+    r.c2 = P.dTchia2dX * htu;
+    r.a2 = -P.dTchia2dX * d.Htdu[(i - 1) * W + j];
+    r.d2 = d.z[i * W + j] - P.dTchia2dY * (d.Htdv[i * W + j] * v[i * W + j] - d.Htdv[i * W + j - 1] * v[i * W + j - 1]);
+    return r;
+}
+
+/* _vzSolver_calculate_preindex, UVZSolver_multithread.cu:295-350, cell (i, j). */
+__device__ Pre pre_vz(const HdDev& d, int i, int j)
+{
+    const hd_params& P = d.P;
+    int W = d.W;
+    const double *u = d.u, *v = d.v, *Hm = d.H_moi, *VIS = d.VISCOIDX;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    double vc = v[i * W + j];
+    double utb = (u[(i - 1) * W + j] + u[i * W + j] + u[(i - 1) * W + j + 1] + u[i * W + j + 1]) * 0.25;
+    double vis = VIS[i * W + j];
+    double htv = d.Htdv[i * W + j];
+    r.f1 = P.dTchia2dY * vc + vis * P.dT / P.dYbp;
+    r.f2 = -(2 + d.Ky1[i * W + j] * P.dT * sqrt(vc * vc + utb * utb) / htv + (2 * P.dT * vis) / P.dYbp);
+    r.f3 = P.dT * vis / P.dYbp - P.dTchia2dY * vc;
+    if (Hm[(i - 1) * W + j] <= P.H_TINH) {
+        if (utb < 0) {
+            q = utb * (-3 * vc + 4 * v[(i + 1) * W + j] - v[(i + 2) * W + j]) / P.dX2;
+            p = (vc - 2 * v[(i + 1) * W + j] + v[(i + 2) * W + j]) / P.dXbp;
+        }
+    } else {
+        if (Hm[(i + 1) * W + j] <= P.H_TINH) {
+            if ((Hm[(i - 2) * W + j] > P.H_TINH) && (utb > 0)) {
+                q = utb * (3 * vc - 4 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / P.dX2;
+                p = (vc - 2 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / P.dXbp;
+            }
+        } else {
+            q = utb * (v[(i + 1) * W + j] - v[(i - 1) * W + j]) / P.dX2;
+            p = (v[(i + 1) * W + j] - 2 * vc + v[(i - 1) * W + j]) / P.dXbp;
+        }
+    }
+    r.f5 = -2 * vc + P.dT * q + P.dT * P.CORIOLIS_FORCE * utb - P.dT * vis * p - P.dT * (P.Windy - d.Tsyw[i * W + j]) / htv;
+    r.c2 = P.dTchia2dY * htv;
+    r.a2 = -P.dTchia2dY * d.Htdv[i * W + j - 1];
+    r.d2 = d.z[i * W + j] - P.dTchia2dX * (d.Htdu[i * W + j] * u[i * W + j] - d.Htdu[(i - 1) * W + j] * u[(i - 1) * W + j]);
+    return r;
+}
+
+enum { S_WALL = 0, S_Q = 1, S_OPEN = 2 };
+enum { E_Q = 1, E_OPEN = 2, E_WALL = 3 };
+
+/* Start/end conditions of one run, _calculate_matrix_coeff
+ * (UVZSolver_multithread.cu:469-593): start kind, end kind and system size
+ * sn.  dkQ1 / dkQ2 are the Q flags of the start / end edges; at a non-wall
+ * start the end test reads dkQ1, as UVZSolver_multithread.cu:562 is written. */
+__device__ __forceinline__ void run_kinds(bool br1, bool br2, bool dkQ1, bool dkQ2, int first, int last, int dkfr,
+                                          int& skind, int& ekind, int& sn)
+{
+    sn = 2 * (last - first);
+    if (br1) {
+        skind = S_WALL;
+        if (!br2) {
+            if (dkQ2 && last == dkfr) ekind = E_Q;
+            else { sn--; ekind = E_OPEN; }
+        } else ekind = E_WALL;
+    } else {
+        bool isBienran = dkQ1 && (first == 2);
+        skind = isBienran ? S_Q : S_OPEN;
+        if (!br2) {
+            if (dkQ1 && last == dkfr) { if (!isBienran) sn--; ekind = E_Q; }
+            else { sn--; if (!isBienran) sn -= 1; ekind = E_OPEN; }
+        } else { if (!isBienran) sn--; ekind = E_WALL; }
+    }
+}
+
+struct Mom {
+    double a1, b1, c1, d1;
+};
+
+/* _calculate_abcd, UVZSolver_multithread.cu:409-466, at position k of the run
+ * [first, last], given the preindex values of k and k+1.  The wall overrides
+ * (first / last-1) replace the general formula. */
+__device__ __forceinline__ Mom abcd(const Pre& a, const Pre& b, double f4, int k, int first, int last, bool br1, bool br2)
+{
+    Mom m;
+    if (last - 1 > first) {
+        if (br1 && k == first) {
+            m.a1 = f4;
+            m.b1 = a.f2 - (a.f3 * b.a2 / b.c2);
+            m.c1 = -f4 - (a.f3 / b.c2);
+            m.d1 = a.f5 - (a.f3 * b.d2 / b.c2);
+        } else if (br2 && k == last - 1) {
+            m.a1 = f4 - a.f1 / a.a2;
+            m.b1 = a.f2 - (a.f1 * a.c2 / a.a2);
+            m.c1 = -f4;
+            m.d1 = a.f5 - (a.f1 * a.d2 / a.a2);
+        } else {
+            m.a1 = f4 - a.f1 / a.a2;
+            m.c1 = -f4 - (a.f3 / b.c2);
+            m.b1 = a.f2 - (a.f1 * a.c2 / a.a2) - (a.f3 * b.a2 / b.c2);
+            m.d1 = a.f5 - (a.f1 * a.d2 / a.a2) - (a.f3 * b.d2 / b.c2);
+        }
+    } else {
+        m.a1 = f4;
+        m.b1 = a.f2;
+        m.d1 = a.f5;
+        m.c1 = -f4;
+    }
+    return m;
+}
+
+/* Locate the run of `line` holding position k in an LDS copy of the table. */
+__device__ __forceinline__ int find_run(const int* dau, const int* cuoi, int n, int k, int& first, int& last)
+{
+    for (int s = 0; s < n; ++s) {
+        int a = dau[s], b = cuoi[s];
+        if (a <= k && k <= b) { first = a; last = b; return s; }
+    }
+    return -1;
+}
+
+/* Wall flags of locate_segment_u (column line) / locate_segment_v (row line),
+ * constant.cuh:135-178. */
+__device__ __forceinline__ void wall_flags(const HdDev& d, bool isU, int line, int first, int last, bool& br1, bool& br2)
+{
+    int W = d.W;
+    const double* h = d.h;
+    double ND = d.P.NANGDAY;
+    if (isU) {
+        br1 = (first > 2) || ((first == 2) && ((h[1 * W + line] + h[1 * W + line - 1]) * 0.5 == ND));
+        br2 = (last < d.N) || ((last == d.N) && ((h[d.N * W + line] + h[d.N * W + line - 1]) * 0.5 == ND));
+    } else {
+        br1 = (first > 2) || ((first == 2) && ((h[line * W + first - 1] + h[(line - 1) * W + first - 1]) * 0.5 == ND));
+        br2 = (last < d.M) || ((last == d.M) && ((h[line * W + last] + h[(line - 1) * W + last]) * 0.5 == ND));
+    }
+}
+
+/* Tridiagonal rows of one wet cell.  isU: UZ (line = column j, position = row
+ * i) or VZ (line = row i, position = column j).  Rows of run [f, l] sit at
+ * absolute index 2k (+1) of the line, as in the reference (offset = 2 first).
+ * Layout: wall/Q start: row 2k = continuity of k, 2k+1 = momentum of k; open
+ * start: 2k = momentum of k, 2k+1 = continuity of k+1. */
+template <bool isU>
+__global__ void __launch_bounds__(256) k_adi_coef(HdDev d, int sidx, int eidx)
+{
+    __shared__ int s_moc[64];
+    __shared__ int s_dau[64][24];
+    __shared__ int s_cuoi[64][24];
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    int j0 = blockIdx.x * 64;
+    int i0 = blockIdx.y * 16;
+    /* lines held in LDS: columns j0..j0+63 (UZ) or rows i0..i0+15 (VZ) */
+    int tid = ty * 64 + tx;
+    if (isU) {
+        for (int t = tid; t < 64; t += 256) {
+            int L = j0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx && L <= M + 2) ? min(d.mocj[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 64 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.dauj[(j0 + c) * SL + s]; s_cuoi[c][s] = d.cuoij[(j0 + c) * SL + s]; }
+        }
+    } else {
+        for (int t = tid; t < 16; t += 256) {
+            int L = i0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx && L <= N + 2) ? min(d.moci[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 16 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.daui[(i0 + c) * SL + s]; s_cuoi[c][s] = d.cuoii[(i0 + c) * SL + s]; }
+        }
+    }
+    __syncthreads();
+    int j = j0 + tx;
+    if (j > M + 2) return;
+    const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    for (int r = ty; r < 16; r += 4) {
+        int i = i0 + r;
+        if (i > N + 2) break;
+        int line = isU ? j : i;
+        int k = isU ? i : j;
+        int li = isU ? tx : r;
+        int first, last;
+        int s = find_run(s_dau[li], s_cuoi[li], s_moc[li], k, first, last);
+        if (s < 0) continue;
+        bool br1, br2;
+        wall_flags(d, isU, line, first, last, br1, br2);
+        bool dkQ1 = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0;
+        bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
+        int dkfr = isU ? N : M;
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
+        Pre a = isU ? pre_uz(d, k, line) : pre_vz(d, line, k);
+        Pre b = a;
+        if (k + 1 <= last) b = isU ? pre_uz(d, k + 1, line) : pre_vz(d, line, k + 1);
+        long long seg = (long long)line * SL + s;
+        if (k == first) {
+            d.seg_end[seg * 6 + 0] = a.a2;
+            d.seg_end[seg * 6 + 1] = a.c2;
+            d.seg_end[seg * 6 + 2] = a.d2;
+            d.seg_sn[seg] = sn;
+        }
+        if (k == last) {
+            d.seg_end[seg * 6 + 3] = a.a2;
+            d.seg_end[seg * 6 + 4] = a.c2;
+            d.seg_end[seg * 6 + 5] = a.d2;
+        }
+        if (sn < 0) continue;
+        double ub1 = isU ? d.ubt[line] : d.vbd[line];
+        double ub2 = isU ? d.ubp[line] : d.vbt[line];
+        double TZ_l = isU ? d.t_z[first * W + line] : d.t_z[line * W + first];
+        double TZ_r = isU ? d.t_z[last * W + line] : d.t_z[line * W + last];
+        size_t base = (size_t)line * tw;
+        for (int e = 0; e < 2; ++e) {
+            int rr = 2 * (k - first) + e; /* system row */
+            if (rr > sn) break;
+            double A, B, C, D;
+            bool cont = (skind != S_OPEN) ? (e == 0) : (e == 1);
+            if (cont) {
+                const Pre& c = (skind != S_OPEN) ? a : b;
+                A = c.a2; B = 1; C = c.c2; D = c.d2;
+            } else {
+                Mom m = abcd(a, b, f4, k, first, last, br1, br2);
+                A = m.a1; B = m.b1; C = m.c1; D = m.d1;
+            }
+            if (rr == 0) {
+                if (skind == S_Q) D = a.d2 - a.a2 * ub1;
+                else if (skind == S_OPEN) D = D - A * TZ_l;
+            }
+            if (rr == sn) {
+                if (ekind == E_Q) { /* row is the continuity row of `last` */
+                    const Pre& c = (k == last) ? a : b;
+                    A = c.a2; B = 1; D = c.d2 - c.c2 * ub2;
+                } else if (ekind == E_OPEN) { /* momentum row of last-1 (this cell) */
+                    Mom m = abcd(a, b, f4, k, first, last, br1, br2);
+                    A = m.a1; B = m.b1; D = m.d1 - m.c1 * TZ_r;
+                } else {
+                    const Pre& c = (k == last) ? a : b;
+                    A = c.a2; B = 1; D = c.d2;
+                }
+            }
+            size_t o = base + 2 * k + e;
+            d.AA[o] = A; d.BB[o] = B; d.CC[o] = C; d.DD[o] = D;
+        }
+    }
+}
+
+/* tridiagSolver / tridiag (UVZSolver_multithread.cu:203-262) and the
+ * extraction of _uz/_vzSolver_extract_solution (:595-723), one lane per line:
+ * forward elimination over each run, back substitution writing t_z and t_u
+ * (UZ) or t_v (VZ) as each unknown is resolved. */
+template <bool isU>
+__global__ void __launch_bounds__(64) k_adi_solve(HdDev d, int sidx, int eidx)
+{
+    int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
+    if (line > eidx) return;
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    int nseg = isU ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    const int* dau = (isU ? d.dauj : d.daui) + line * SL;
+    const int* cuoi = (isU ? d.cuoij : d.cuoii) + line * SL;
+    bool dkQ1 = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0;
+    bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
+    bool exQ = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;  /* extraction end test: bienQ[3] / bienQ[0] */
+    bool exQs = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0; /* extraction start test: bienQ[2] / bienQ[1] */
+    int dkfr = isU ? N : M;
+    double ub1 = isU ? d.ubt[line] : d.vbd[line];
+    double ub2 = isU ? d.ubp[line] : d.vbt[line];
+    double* tz = d.t_z;
+    double* tq = isU ? d.t_u : d.t_v;
+    /* element (position k) of this line */
+    auto at = [&](int k) -> size_t { return isU ? (size_t)k * W + line : (size_t)line * W + k; };
+    size_t base = (size_t)line * tw;
+    const double *AA = d.AA + base, *BB = d.BB + base, *CC = d.CC + base, *DD = d.DD + base;
+    double *Ap = d.Ap + base, *Bp = d.Bp + base;
+    for (int s = 0; s < nseg; ++s) {
+        int first = dau[s], last = cuoi[s];
+        bool br1, br2;
+        wall_flags(d, isU, line, first, last, br1, br2);
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
+        (void)ekind;
+        const double* se = d.seg_end + ((size_t)line * SL + s) * 6;
+        double q_last_prev = 0; /* t_u/t_v at last-1 after extraction */
+        bool have_prev = false;
+        double zl_sol = 0;
+        if (sn >= 0) {
+            int o = 2 * first;
+            double x;
+            if (sn == 0) {
+                x = DD[o] / BB[o];
+            } else {
+                double ap = -CC[o] / BB[o];
+                double bp = DD[o] / BB[o];
+                Ap[o] = ap; Bp[o] = bp;
+                for (int r = 1; r < sn; ++r) {
+                    double a = AA[o + r], b = BB[o + r], c = CC[o + r], dd = DD[o + r];
+                    double e = a * ap + b;
+                    double nap = -c / e;
+                    double nbp = (dd - (a * bp)) / e;
+                    ap = nap; bp = nbp;
+                    Ap[o + r] = ap; Bp[o + r] = bp;
+                }
+                double a = AA[o + sn];
+                x = (DD[o + sn] - (a * bp)) / (BB[o + sn] + (a * ap));
+            }
+            /* back substitution + extraction, rows sn .. 0 */
+            for (int r = sn; r >= 0; --r) {
+                if (r < sn) x = Bp[o + r] + (Ap[o + r] * x);
+                int k;
+                bool isz;
+                if (skind != S_OPEN) { k = first + (r >> 1); isz = (r & 1) == 0; }
+                else { isz = (r & 1) == 1; k = first + ((r + 1) >> 1); }
+                if (k < last) {
+                    if (isz) tz[at(k)] = x;
+                    else {
+                        tq[at(k)] = x;
+                        if (k == last - 1) { q_last_prev = x; have_prev = true; }
+                    }
+                } else if (isz && r == sn) {
+                    zl_sol = x;
+                }
+            }
+        }
+        /* start of the run */
+        if (first < last) {
+            if (br1) tq[at(first - 1)] = 0;
+            else if (exQs && first == 2) tq[at(first - 1)] = ub1;
+            else if (sn >= 0) {
+                double a2f = se[0], c2f = se[1], d2f = se[2];
+                tq[at(first - 1)] = (d2f - tz[at(first)] - c2f * tq[at(first)]) / a2f;
+            }
+        }
+        /* end of the run */
+        if (br2) {
+            tq[at(last)] = 0;
+            if (sn >= 0) tz[at(last)] = zl_sol;
+        } else if (exQ && last == dkfr) {
+            tq[at(last)] = ub2;
+            if (sn >= 0) tz[at(last)] = zl_sol;
+        } else {
+            double prev = have_prev ? q_last_prev : tq[at(last - 1)];
+            double a2l = se[3], c2l = se[4], d2l = se[5];
+            tq[at(last)] = (d2l - a2l * prev - tz[at(last)]) / c2l;
+        }
+    }
+}
+
+/* ================================================================ smoothing */
+
+/* Normalize + update_buffer, support_funcs.cu:682-729, writing the partner
+ * buffer (the caller swaps them).  With `post` set it also does
+ * update_h_moi (support_funcs.cu:60-73), the drying test of Reset_states
+ * (support_funcs.cu:114-127 / 173-185) for every cell, and records the
+ * positions where the serial wetting pass can act (a neighbour along the
+ * reset direction has khouot == 1). */
+__global__ void __launch_bounds__(256) k_normalize(HdDev d, int isU, int post, int horizontal, int* cand_cnt,
+                                                    int* cand, unsigned char* dryflag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    int N = d.N, M = d.M, W = d.W;
+    if (j >= M + 3) return;
+    int p = i * W + j;
+    const double* val = isU ? d.t_u : d.t_v;
+    double* out = isU ? d.tmp_u : d.tmp_v;
+    int close = isU ? W : 1, far = isU ? 1 : W;
+    double r = val[p];
+    if (i >= 2 && i <= N && j >= 2 && j <= M && r != 0) {
+        int count = 2;
+        double nb = val[p - close] + val[p + close];
+        if (d.khouot[p - far] == 0) { nb += val[p - far]; count++; }
+        if (d.khouot[p + far] == 0) { nb += val[p + far]; count++; }
+        r = r * d.P.heso + (1 - d.P.heso) * nb / count;
+    }
+    out[p] = r;
+    if (!post) return;
+    int kh = d.khouot[p];
+    unsigned char dry = 0;
+    if (i >= 1 && i <= N && j >= 1 && j <= M && kh == 0) {
+        double hm = d.htaiz[p] + d.t_z[p];
+        d.H_moi[p] = hm;
+        bool inrange = horizontal ? (j >= 2) : (i >= 2);
+        dry = inrange && (hm <= d.P.H_TINH);
+    }
+    dryflag[p] = dry;
+    if (horizontal) {
+        if (i >= 1 && i <= N && j >= 2 && j <= M && (d.khouot[p - 1] == 1 || d.khouot[p + 1] == 1)) {
+            int slot = atomicAdd(&cand_cnt[i], 1);
+            if (slot < kCand) cand[i * kCand + slot] = j;
+        }
+    } else {
+        if (j >= 1 && j <= M && i >= 2 && i <= N && (d.khouot[p - W] == 1 || d.khouot[p + W] == 1)) {
+            int slot = atomicAdd(&cand_cnt[j], 1);
+            if (slot < kCand) cand[j * kCand + slot] = i;
+        }
+    }
+}
+
+/* ================================================================ explicit */
+
+/* vSolver, UVZSolver_multithread.cu:727-768, at (row, col); reads the
+ * smoothed t_u. */
+__device__ double v_solver(const HdDev& d, int row, int col)
+{
+    const hd_params& P = d.P;
+    int W = d.W;
+    const double *v = d.v, *u = d.u, *t_u = d.t_u, *z = d.z, *Hm = d.H_moi;
+    double p, q, tmp;
+    q = 0; p = 0; tmp = 0;
+    double vc = v[row * W + col];
+    double utb = (u[(row - 1) * W + col] + u[row * W + col] + u[(row - 1) * W + col + 1] + u[row * W + col + 1]) * 0.25;
+    double t_utb = (t_u[(row - 1) * W + col] + t_u[row * W + col] + t_u[(row - 1) * W + col + 1] + t_u[row * W + col + 1]) * 0.25;
+    double htv = d.Htdv[row * W + col];
+    p = (v[row * W + col + 1] - v[row * W + col - 1]) / P.dY2;
+    p = P.HaiChiadT + p + d.Ky1[row * W + col] * sqrt(utb * utb + vc * vc) / htv;
+    if (Hm[(row - 1) * W + col] <= P.H_TINH) {
+        if (utb < 0) {
+            q = t_utb * (-3 * vc + 4 * v[(row + 1) * W + col] - v[(row + 2) * W + col]) / P.dX2;
+            tmp = (vc - 2 * v[(row + 1) * W + col] + v[(row + 2) * W + col]) / P.dXbp;
+        }
+    } else {
+        if (Hm[(row + 1) * W + col] <= P.H_TINH) {
+            if ((Hm[(row - 2) * W + col] > P.H_TINH) && (utb > 0)) {
+                q = t_utb * (3 * vc - 4 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / P.dX2;
+                tmp = (vc - 2 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / P.dXbp;
+            }
+        } else {
+            q = t_utb * (v[(row + 1) * W + col] - v[(row - 1) * W + col]) / P.dX2;
+            tmp = (v[(row + 1) * W + col] - 2 * vc + v[(row - 1) * W + col]) / P.dXbp;
+        }
+    }
+    q = P.HaiChiadT * vc - q - P.CORIOLIS_FORCE * t_utb;
+    q = (q - P.g * (z[row * W + col + 1] - z[row * W + col]) / P.dY
+         + d.VISCOIDX[row * W + col] * (tmp + (v[row * W + col + 1] - 2 * vc + v[row * W + col - 1]) / P.dYbp))
+        + (P.Windy - d.Tsyw[row * W + col]) / htv;
+    return q / p;
+}
+
+/* uSolver, UVZSolver_multithread.cu:773-815, at (row, col); reads the
+ * smoothed t_v. */
+__device__ double u_solver(const HdDev& d, int row, int col)
+{
+    const hd_params& P = d.P;
+    int W = d.W;
+    const double *v = d.v, *u = d.u, *t_v = d.t_v, *z = d.z, *Hm = d.H_moi;
+    double p, q, tmp;
+    p = 0; q = 0; tmp = 0;
+    double uc = u[row * W + col];
+    double vtb = (v[row * W + col - 1] + v[row * W + col] + v[(row + 1) * W + col - 1] + v[(row + 1) * W + col]) * 0.25;
+    double t_vtb = (t_v[row * W + col - 1] + t_v[row * W + col] + t_v[(row + 1) * W + col - 1] + t_v[(row + 1) * W + col]) * 0.25;
+    double htu = d.Htdu[row * W + col];
+    p = (u[(row + 1) * W + col] - u[(row - 1) * W + col]) / P.dX2;
+    p = (P.HaiChiadT + p + d.Kx1[row * W + col] * sqrt(vtb * vtb + uc * uc) / htu);
+    if (Hm[row * W + col - 1] <= P.H_TINH) {
+        if (vtb < 0) {
+            q = t_vtb * (-3 * uc + 4 * u[row * W + col + 1] - u[row * W + col + 2]) / P.dY2;
+            tmp = (uc - 2 * u[row * W + col + 1] + u[row * W + col + 2]) / P.dYbp;
+        }
+    } else {
+        if (Hm[row * W + col + 1] <= P.H_TINH) {
+            if ((Hm[row * W + col - 2] > P.H_TINH) && (vtb > 0)) {
+                q = t_vtb * (3 * uc - 4 * u[row * W + col - 1] + u[row * W + col - 2]) / P.dY2;
+                tmp = (uc - 2 * u[row * W + col - 1] + u[row * W + col - 2]) / P.dYbp;
+            }
+        } else {
+            q = t_vtb * (u[row * W + col + 1] - u[row * W + col - 1]) / P.dY2;
+            tmp = (u[row * W + col + 1] - 2 * uc + u[row * W + col - 1]) / P.dYbp;
+        }
+    }
+    q = P.HaiChiadT * uc - q + P.CORIOLIS_FORCE * t_vtb;
+    q = (q - P.g * (z[(row + 1) * W + col] - z[row * W + col]) / P.dX
+         + d.VISCOIDX[row * W + col] * (tmp + (u[(row + 1) * W + col] - 2 * uc + u[(row - 1) * W + col]) / P.dXbp))
+        + (P.Windx - d.Tsxw[row * W + col]) / htu;
+    return q / p;
+}
+
+/* solveV (UVZSolver_multithread.cu:1055-1071) or solveU (:947-968): one
+ * thread per cell, the cell's run from an LDS copy of the line tables; cells
+ * first <= k < last of runs with first < last. */
+template <bool isU>
+__global__ void __launch_bounds__(256) k_explicit(HdDev d, int sidx, int eidx)
+{
+    __shared__ int s_moc[64];
+    __shared__ int s_dau[64][24];
+    __shared__ int s_cuoi[64][24];
+    int N = d.N, M = d.M, SL = d.SL;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    int j0 = blockIdx.x * 64, i0 = blockIdx.y * 16;
+    int tid = ty * 64 + tx;
+    /* solveU: column lines (dauj); solveV: row lines (daui) */
+    if (isU) {
+        for (int t = tid; t < 64; t += 256) {
+            int L = j0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx && L <= M + 2) ? min(d.mocj[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 64 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.dauj[(j0 + c) * SL + s]; s_cuoi[c][s] = d.cuoij[(j0 + c) * SL + s]; }
+        }
+    } else {
+        for (int t = tid; t < 16; t += 256) {
+            int L = i0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx && L <= N + 2) ? min(d.moci[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 16 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.daui[(i0 + c) * SL + s]; s_cuoi[c][s] = d.cuoii[(i0 + c) * SL + s]; }
+        }
+    }
+    __syncthreads();
+    int j = j0 + tx;
+    if (j > M) return;
+    for (int r = ty; r < 16; r += 4) {
+        int i = i0 + r;
+        if (isU) { if (i < 1 || i > N) continue; }
+        else { if (i > N) break; }
+        int li = isU ? tx : r;
+        int k = isU ? i : j;
+        int first, last;
+        if (find_run(s_dau[li], s_cuoi[li], s_moc[li], k, first, last) < 0) continue;
+        if (isU) {
+            if (first > last || k < first - 1 || k >= last) continue;
+            d.t_u[i * d.W + j] = u_solver(d, i, j);
+        } else {
+            if (first >= last || k >= last || k < first) continue;
+            d.t_v[i * d.W + j] = v_solver(d, i, j);
+        }
+    }
+}
+
+/* update_margin_elem_V (UVZSolver_multithread.cu:1008-1052) or
+ * update_margin_elem_U (:970-1006): one lane per line, runs in order. */
+template <bool isU>
+__global__ void k_margins(HdDev d, int sidx, int eidx)
+{
+    int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
+    if (line > eidx) return;
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const double* h = d.h;
+    const double eps = 1e-16; /* UVZSolver_multithread.cu:12 */
+    double ND = d.P.NANGDAY;
+    int nseg = isU ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    for (int k = 0; k < nseg; ++k) {
+        if (isU) {
+            int j = line;
+            int first = d.dauj[j * SL + k], last = d.cuoij[j * SL + k];
+            bool br1 = (first > 2) || ((first == 2) && ((fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - ND) < eps));
+            bool br2 = (last < N) || ((last == N) && (fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - ND) < eps));
+            double* t = d.t_u;
+            if (br1) t[(first - 1) * W + j] = 0;
+            else t[(first - 1) * W + j] = 2 * t[first * W + j] - t[(first + 1) * W + j];
+            if (br2) t[last * W + j] = 0;
+            else t[last * W + j] = 2 * t[(last - 1) * W + j] - t[(last - 2) * W + j];
+        } else {
+            int i = line;
+            int first = d.daui[i * SL + k], last = d.cuoii[i * SL + k];
+            bool br1 = (first > 2) || ((first == 2) && (fabs((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 - ND) < eps));
+            bool br2 = (last < M) || ((last == M) && (fabs((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 - ND) < eps));
+            double* t = d.t_v;
+            if (br1) t[i * W + first - 1] = 0;
+            else t[i * W + first - 1] = 2 * t[i * W + first] - t[i * W + first + 1];
+            if (br2) t[i * W + last] = 0;
+            else t[i * W + last] = 2 * t[i * W + last - 1] - t[i * W + last - 2];
+        }
+    }
+}
+
+/* ================================================================ wet / dry */
+
+/* Wetting pass of Reset_states_horizontal (support_funcs.cu:92-112) /
+ * _vertical (:152-171), one lane per line.  The serial scan can only act at
+ * positions next to a khouot == 1 cell; those were recorded by k_normalize
+ * and are visited here in increasing order, re-checking the current state, so
+ * cascades along the line happen exactly as in the serial loop.  A line with
+ * more candidates than fit the list is scanned in full. */
+__global__ void k_wetting(HdDev d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag)
+{
+    int line = blockIdx.x * blockDim.x + threadIdx.x;
+    int N = d.N, M = d.M, W = d.W;
+    int lo = horizontal ? 1 : 1, hi = horizontal ? N : M;
+    if (line < lo || line > hi) return;
+    int n = cand_cnt[line];
+    if (n == 0) return;
+    cand_cnt[line] = 0;
+    int list[kCand];
+    bool full = n > kCand;
+    int cnt = full ? 0 : n;
+    if (!full) {
+        for (int t = 0; t < n; ++t) list[t] = cand[line * kCand + t];
+        for (int a = 1; a < n; ++a) { /* insertion sort */
+            int v = list[a], b = a - 1;
+            while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; --b; }
+            list[b + 1] = v;
+        }
+    }
+    int pmax = horizontal ? M : N;
+    double *tz = d.t_z, *Hm = d.H_moi, *FS = d.FS;
+    const double *z = d.z, *ht = d.htaiz;
+    int* kh = d.khouot;
+    auto at = [&](int k) -> int { return horizontal ? line * W + k : k * W + line; };
+    auto open = [&](int q, int src) {
+        int pq = at(q), ps = at(src);
+        tz[pq] = tz[ps];
+        Hm[pq] = ht[pq] + tz[ps];
+        kh[pq] = 0;
+        FS[pq] = FS[ps];
+        bool inrange = horizontal ? (q >= 2 && q <= M && line >= 1 && line <= N) : (q >= 2 && q <= N && line >= 1 && line <= M);
+        dryflag[pq] = inrange && (Hm[pq] <= d.P.H_TINH);
+    };
+    int last_done = -1;
+    int total = full ? (pmax - 1) : cnt;
+    for (int t = 0; t < total; ++t) {
+        int pos = full ? (2 + t) : list[t];
+        if (pos == last_done) continue;
+        last_done = pos;
+        int pp = at(pos);
+        if (tz[pp] > z[pp]) {
+            if (kh[at(pos - 1)] == 1) open(pos - 1, pos);
+            if (kh[at(pos + 1)] == 1) open(pos + 1, pos);
+        }
+    }
+}
+
+/* update_uvz (support_funcs.cu:671-680) fused with the drying writes of
+ * Reset_states (support_funcs.cu:114-127 / 173-185) and Htuongdoi
+ * (support_funcs.cu:338-354).  Drying cell (a,b) zeroes t_u at (a-1,b),(a,b)
+ * and t_v at (a,b-1),(a,b); each cell applies the flags of itself and of the
+ * neighbours that write it, then copies.  Face depths read the new z (t_z)
+ * inside [0,N]x[0,M] and the untouched z outside, as after update_uvz. */
+__global__ void __launch_bounds__(256) k_uvz(HdDev d, int kd, int kn, const unsigned char* dryflag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    int N = d.N, M = d.M, W = d.W;
+    if (j >= M + 3) return;
+    int p = i * W + j;
+    bool me = dryflag[p];
+    bool dn = (i + 1 <= N + 2) ? dryflag[p + W] : false;
+    bool rt = (j + 1 <= M + 2) ? dryflag[p + 1] : false;
+    double tu = d.t_u[p], tv = d.t_v[p];
+    if (me || dn) { tu = 0; d.t_u[p] = 0; }
+    if (me || rt) { tv = 0; d.t_v[p] = 0; }
+    if (me) { d.khouot[p] = 1; d.FS[p] = 0; }
+    if (i <= N && j <= M) {
+        d.z[p] = d.t_z[p];
+        d.u[p] = tu * (1 - kd);
+        d.v[p] = tv * (1 - kn);
+    }
+    if (i >= 1 && i <= N && j >= 1 && j <= M) {
+        const double* h = d.h;
+        double zc = d.t_z[p];
+        double zd = (i + 1 <= N) ? d.t_z[p + W] : d.z[p + W];
+        double zr = (j + 1 <= M) ? d.t_z[p + 1] : d.z[p + 1];
+        d.Htdu[p] = (h[i * W + j - 1] + h[p] + zd + zc) * 0.5;
+        d.Htdv[p] = (h[(i - 1) * W + j] + h[p] + zr + zc) * 0.5;
+    }
+}
+
+/* ================================================================ segments */
+
+/* Interpolate_FS_ng, support_funcs.cu:189-208, one column j of the row
+ * `location` (source row clamped to [0, N+1]; zero divisor -> quotient 0). */
+__device__ void fs_ng(const HdDev& d, int location, int offset, int sign, int j)
+{
+    int W = d.W;
+    if (d.t_u[location * W + j] > 0) {
+        double deltaX = (d.u[location * W + j] + d.t_u[location * W + j]) * 0.25 * d.P.dT;
+        int i = (int)(offset + sign * (deltaX / d.P.dX));
+        if (i < 0) i = 0;
+        if (i > d.N + 1) i = d.N + 1;
+        int i_deltaX = hd_rint_int(deltaX);
+        int i_dX = hd_rint_int(d.P.dX);
+        int q = i_dX != 0 ? i_deltaX / i_dX : 0;
+        d.FS[location * W + j] = d.FS[i * W + j] + sign * (d.FS[(i + 1) * W + j] - d.FS[i * W + j]) * (deltaX / d.P.dX - q);
+    }
+}
+
+/* Interpolate_FS_d, support_funcs.cu:210-229, one row i of column `location`
+ * (source column clamped to [0, M+1]). */
+__device__ void fs_d(const HdDev& d, int location, int offset, int sign, int i)
+{
+    int W = d.W;
+    if (d.t_v[i * W + location] > 0) {
+        double deltaY = (d.v[i * W + location] + d.t_v[i * W + location]) * 0.25 * d.P.dT;
+        int j = (int)(offset + sign * (deltaY / d.P.dY));
+        if (j < 0) j = 0;
+        if (j > d.M + 1) j = d.M + 1;
+        int j_deltaY = hd_rint_int(deltaY);
+        int j_dY = hd_rint_int(d.P.dY);
+        int q = j_dY != 0 ? j_deltaY / j_dY : 0;
+        d.FS[i * W + location] = d.FS[i * W + j + 1] + (d.FS[i * W + j + 1] - d.FS[i * W + j]) * (deltaY / d.P.dY - q);
+    }
+}
+
+/* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335).
+ * A wet run within [2, lim] starting at a <= lim-2 is a segment; its end is
+ * its last wet cell (the serial scan's rules, including dropping a run that
+ * starts at lim-1).  Blocks: [0, nrowb) one wave per row (ballots along the
+ * row), [nrowb, nrowb+ncolb) one wave per column (lanes along the column),
+ * last block: the FS edge interpolations (rows N and 2, then column M, as the
+ * two reference kernels order them).  With count_wet the row waves also add
+ * the row's khouot == 0 cells to the wet-cell accumulator. */
+__global__ void __launch_bounds__(64) k_find_limits(HdDev d, int nrowb, int ncolb, int count_wet)
+{
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int lane = threadIdx.x;
+    const int* kh = d.khouot;
+    int b = blockIdx.x;
+    if (b < nrowb) {
+        int i = b; /* rows 0..N+2 for counting, 2..N for segments */
+        if (count_wet) {
+            int c = 0;
+            for (int j = lane; j < M + 3; j += kWave) c += (kh[i * W + j] == 0);
+            for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
+            if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
+        }
+        if (i < 2 || i > N) return;
+        const int* row = kh + i * W;
+        auto wet = [&](int x) -> bool { return x >= 2 && x <= M && row[x] == 0; };
+        int ns = 0, ne = 0;
+        for (int c0 = 2; c0 <= M; c0 += kWave) {
+            int j = c0 + lane;
+            bool w0 = wet(j);
+            bool st = w0 && (j == 2 || !wet(j - 1)) && j <= M - 2;
+            bool en = w0 && !wet(j + 1) &&
+                      (j <= M - 2 || (j == M - 1 && wet(M - 2)) || (j == M && wet(M - 1) && wet(M - 2)));
+            unsigned long long sm = __ballot(st), em = __ballot(en);
+            if (st) {
+                int k = ns + __popcll(sm & lanemask_lt());
+                if (k < SL) d.daui[i * SL + k] = j;
+            }
+            if (en) {
+                int k = ne + __popcll(em & lanemask_lt());
+                if (k < SL) d.cuoii[i * SL + k] = j;
+            }
+            ns += __popcll(sm);
+            ne += __popcll(em);
+        }
+        if (lane == 0) {
+            d.moci[i] = ns < SL ? ns : SL;
+            if (ns > SL) d.flags[0] = 1;
+        }
+        return;
+    }
+    b -= nrowb;
+    if (b < ncolb) {
+        int j = 2 + b;
+        if (j > M) return;
+        auto wet = [&](int x) -> bool { return x >= 2 && x <= N && kh[x * W + j] == 0; };
+        int ns = 0, ne = 0;
+        for (int c0 = 2; c0 <= N; c0 += kWave) {
+            int i = c0 + lane;
+            bool w0 = wet(i);
+            bool st = w0 && (i == 2 || !wet(i - 1)) && i <= N - 2;
+            bool en = w0 && !wet(i + 1) &&
+                      (i <= N - 2 || (i == N - 1 && wet(N - 2)) || (i == N && wet(N - 1) && wet(N - 2)));
+            unsigned long long sm = __ballot(st), em = __ballot(en);
+            if (st) {
+                int k = ns + __popcll(sm & lanemask_lt());
+                if (k < SL) d.dauj[j * SL + k] = i;
+            }
+            if (en) {
+                int k = ne + __popcll(em & lanemask_lt());
+                if (k < SL) d.cuoij[j * SL + k] = i;
+            }
+            ns += __popcll(sm);
+            ne += __popcll(em);
+        }
+        if (lane == 0) {
+            d.mocj[j] = ns < SL ? ns : SL;
+            if (ns > SL) d.flags[0] = 1;
+        }
+        return;
+    }
+    /* FS edge interpolation block (runs with 256 threads requested by host) */
+}
+
+__global__ void __launch_bounds__(256) k_fs_edges(HdDev d)
+{
+    int N = d.N, M = d.M;
+    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
+        fs_ng(d, N, N - 1, -1, j);
+    }
+    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
+        fs_ng(d, 2, 2, 1, j);
+    }
+    __syncthreads();
+    int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
+    for (int i = 1 + threadIdx.x; i <= imax; i += blockDim.x) fs_d(d, M, M - 1, -1, i);
+}
+
+__global__ void k_wet_count(HdDev d)
+{
+    int i = blockIdx.x;
+    int lane = threadIdx.x;
+    int c = 0;
+    for (int j = lane; j < d.M + 3; j += kWave) c += (d.khouot[i * d.W + j] == 0);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
+    if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
+}
+
+/* ================================================================ batched Thomas */
+
+__global__ void k_thomas_batch(int nsys, const int* off, const int* sn, const double* A, const double* B,
+                               const double* C, const double* D, double* x, double* Ap, double* Bp)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    int o = off[s], n = sn[s];
+    if (n < 0) return;
+    if (n == 0) { x[o] = D[o] / B[o]; return; }
+    double ap = -C[o] / B[o], bp = D[o] / B[o];
+    Ap[o] = ap; Bp[o] = bp;
+    for (int r = 1; r < n; ++r) {
+        double e = A[o + r] * ap + B[o + r];
+        double nap = -C[o + r] / e;
+        double nbp = (D[o + r] - (A[o + r] * bp)) / e;
+        ap = nap; bp = nbp;
+        Ap[o + r] = ap; Bp[o + r] = bp;
+    }
+    double xv = (D[o + n] - (A[o + n] * bp)) / (B[o + n] + (A[o + n] * ap));
+    x[o + n] = xv;
+    for (int r = n - 1; r >= 0; --r) { xv = Bp[o + r] + (Ap[o + r] * xv); x[o + r] = xv; }
+}
+
+} // namespace
+
+/* ================================================================ launchers */
+
+static inline dim3 grid2d(const HdDev& d, int bx) { return dim3((d.M + 3 + bx - 1) / bx, d.N + 3); }
+
+void hd_launch_onetime_init(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_onetime_init, grid2d(d, 128), dim3(128), 0, s, d);
+}
+
+void hd_launch_htuongdoi(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_htuongdoi, dim3((d.M + 127) / 128, d.N), dim3(128), 0, s, d);
+}
+
+void hd_launch_preprocess(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_preprocess, dim3(1), dim3(64), 0, s, d);
+}
+
+void hd_launch_find_limits(const HdDev& d, hipStream_t s)
+{
+    hd_launch_segments(d, 0, s);
+    hd_launch_fs_edges(d, s);
+}
+
+void hd_launch_segments(const HdDev& d, int count, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_find_limits, dim3((d.N + 3) + (d.M - 1)), dim3(64), 0, s, d, d.N + 3, d.M - 1, count);
+}
+
+void hd_launch_fs_edges(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fs_edges, dim3(1), dim3(256), 0, s, d);
+}
+
+void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, s, d, tdev, analytic);
+}
+
+void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
+    if (eidx < sidx) return;
+    if (isU) hipLaunchKernelGGL(k_adi_coef<true>, grd, blk, 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_adi_coef<false>, grd, blk, 0, s, d, sidx, eidx);
+}
+
+void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+{
+    int nl = eidx - sidx + 1;
+    if (nl <= 0) return;
+    if (isU) hipLaunchKernelGGL(k_adi_solve<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_adi_solve<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+}
+
+void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
+                              unsigned char* dryflag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_normalize, grid2d(d, 256), dim3(256), 0, s, d, isU, post, horizontal, cand_cnt, cand, dryflag);
+}
+
+void hd_launch_explicit(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
+    if (isU) hipLaunchKernelGGL(k_explicit<true>, grd, blk, 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_explicit<false>, grd, blk, 0, s, d, sidx, eidx);
+}
+
+void hd_launch_margins(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+{
+    int nl = eidx - sidx + 1;
+    if (nl <= 0) return;
+    if (isU) hipLaunchKernelGGL(k_margins<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_margins<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+}
+
+void hd_launch_wetting(const HdDev& d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag, hipStream_t s)
+{
+    int n = (horizontal ? d.N : d.M) + 1;
+    hipLaunchKernelGGL(k_wetting, dim3((n + 63) / 64), dim3(64), 0, s, d, horizontal, cand_cnt, cand, dryflag);
+}
+
+void hd_launch_uvz_dry(const HdDev& d, int kd, int kn, const unsigned char* dryflag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_uvz, grid2d(d, 256), dim3(256), 0, s, d, kd, kn, dryflag);
+}
+
+void hd_launch_wet_count(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wet_count, dim3(d.N + 3), dim3(64), 0, s, d);
+}
+
+int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
+                           const double* C, const double* D, double* x, double* Ap, double* Bp, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_thomas_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap, Bp);
+    return 0;
+}

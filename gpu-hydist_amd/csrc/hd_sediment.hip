@@ -1,0 +1,414 @@
+/*
+ * hd_sediment.hip -- suspended-sediment sweep and bed change
+ * (Sediment_Transport.cu), gfx950.
+ *
+ * Per half-step with t >= sediment_start: VTH + hesoK (one thread per cell),
+ * the concentration rows of the sweep direction (one thread per interior cell
+ * of each wet run), one lane per line for the Thomas solve with the
+ * extraction into tFS, then FS = tFS.  Bed change: Calculate_Qb then
+ * _bed_load, one thread per cell.  Formulas follow the cited reference lines
+ * in the same operation order (contraction off).
+ */
+#include "hd_internal.h"
+
+namespace {
+
+/* source_chs, Sediment_Transport.cu:12-49. */
+__device__ double source_chs(const hd_params& P, double wsm, double Zf, double fs, double fw, double vth, double h_moi)
+{
+    double toee = P.Toe, todd = P.Tod, S = 0;
+    double tob = P.ro * fw * (vth * vth);
+    if (h_moi > P.ghtoe) toee = P.Toe * (1 + P.hstoe * ((h_moi - P.ghtoe)));
+    if (tob > toee) S = P.Mbochat / P.ros * (tob - toee) / toee;
+    else if (tob < todd) {
+        double Pd = 1 - (tob / todd);
+        double beta = 1 + (Zf / (1.25 + 4.75 * hd_pow(Pd, 2.5)));
+        double Cb = beta * fs;
+        S = -wsm * Cb * Pd;
+    }
+    return S;
+}
+
+/* source_nchs, Sediment_Transport.cu:52-60. */
+__device__ double source_nchs(const hd_params& P, double wsm, double Zf, double Uf, double h_moi, double fs)
+{
+    double gamac = 0.434 + 5.975 * Zf + 2.888 * Zf * Zf;
+    double Tx = fmax(0.0, (Uf * Uf - P.Ufr * P.Ufr) / (P.Ufr * P.Ufr));
+    double Cac = 0.015 * P.dm * hd_pow(Tx, 1.5) * hd_pow(P.Dxr, (-0.3)) / (0.05 * h_moi);
+    return wsm * (Cac - gamac * fs);
+}
+
+/* Find_VTH (Sediment_Transport.cu:541-568) and hesoK (:508-539). */
+__global__ void __launch_bounds__(256) k_vth_hesok(HdDev d)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    int N = d.N, M = d.M, W = d.W;
+    const hd_params& P = d.P;
+    if (j > M || i > N) return;
+    int pos = i * W + j;
+    if (i >= 2 && j >= 2) {
+        if ((d.htaiz[pos] > P.NANGDAY) && (d.H_moi[pos] > P.H_TINH)) {
+            double ut = (d.t_u[pos] + d.t_u[pos - W]) * 0.5;
+            double vt = (d.t_v[pos] + d.t_v[pos - 1]) * 0.5;
+            d.VTH[pos] = sqrt(ut * ut + vt * vt);
+        } else d.VTH[pos] = 0;
+    }
+    if (i >= 1 && j >= 1) {
+        double Cz;
+        if (d.Htdu[pos] > 0) {
+            Cz = 18 * hd_log(12 * d.Htdu[pos] / P.Ks);
+            double kx = 5.93 * sqrt(P.g) * (d.h[pos - 1] + d.h[pos]) * 0.25 * fabs(d.t_u[pos]) / Cz;
+            d.Kx[pos] = fmin(100.0, fmax(5.0, kx));
+        }
+        if (d.Htdv[pos] > 0) {
+            Cz = 18 * hd_log(12 * d.Htdv[pos] / P.Ks);
+            double ky = 5.93 * sqrt(P.g) * (d.h[pos - W] + d.h[pos]) * 0.25 * fabs(d.t_v[pos]) / Cz;
+            d.Ky[pos] = fmin(100.0, fmax(5.0, ky));
+        }
+    }
+}
+
+__device__ __forceinline__ void wall_flags2(const HdDev& d, bool isJ, int line, int first, int last, bool& br1, bool& br2)
+{
+    int W = d.W;
+    const double* h = d.h;
+    double ND = d.P.NANGDAY;
+    if (isJ) { /* locate_segment_u, column line */
+        br1 = (first > 2) || ((first == 2) && ((h[1 * W + line] + h[1 * W + line - 1]) * 0.5 == ND));
+        br2 = (last < d.N) || ((last == d.N) && ((h[d.N * W + line] + h[d.N * W + line - 1]) * 0.5 == ND));
+    } else {   /* locate_segment_v, row line */
+        br1 = (first > 2) || ((first == 2) && ((h[line * W + first - 1] + h[(line - 1) * W + first - 1]) * 0.5 == ND));
+        br2 = (last < d.M) || ((last == d.M) && ((h[line * W + last] + h[(line - 1) * W + last]) * 0.5 == ND));
+    }
+}
+
+/* Concentration rows: _FSj_calculate__mactrix_coeff (Sediment_Transport.cu:205-302,
+ * column lines, gamav before Zf and && at the last row, as written) or
+ * _FSi_calculate__mactrix_coeff (:62-165, row lines).  Row of cell k sits at
+ * line*tw + k. */
+template <bool isJ>
+__global__ void __launch_bounds__(256) k_fs_coef(HdDev d, const double* tdev, double s_start, int ketdinh)
+{
+    __shared__ int s_moc[64];
+    __shared__ int s_dau[64][24];
+    __shared__ int s_cuoi[64][24];
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int tx = threadIdx.x, ty = threadIdx.y;
+    int j0 = blockIdx.x * 64, i0 = blockIdx.y * 16;
+    int tid = ty * 64 + tx;
+    int sidx = 3, eidx = isJ ? M - 1 : N - 1;
+    if (isJ) {
+        for (int t = tid; t < 64; t += 256) {
+            int L = j0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx) ? min(d.mocj[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 64 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.dauj[(j0 + c) * SL + s]; s_cuoi[c][s] = d.cuoij[(j0 + c) * SL + s]; }
+        }
+    } else {
+        for (int t = tid; t < 16; t += 256) {
+            int L = i0 + t;
+            s_moc[t] = (L >= sidx && L <= eidx) ? min(d.moci[L], SL) : 0;
+        }
+        __syncthreads();
+        for (int t = tid; t < 16 * SL; t += 256) {
+            int c = t / SL, s = t % SL;
+            if (s < s_moc[c]) { s_dau[c][s] = d.daui[(i0 + c) * SL + s]; s_cuoi[c][s] = d.cuoii[(i0 + c) * SL + s]; }
+        }
+    }
+    __syncthreads();
+    int j = j0 + tx;
+    if (j > M) return;
+    const hd_params& P = d.P;
+    const double t = tdev[0];
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const double *FS = d.FS, *Hm = d.H_moi, *t_u = d.t_u, *t_v = d.t_v, *VTH = d.VTH, *Htdu = d.Htdu, *Htdv = d.Htdv,
+                 *Fw = d.Fw, *Kx = d.Kx, *Ky = d.Ky;
+    for (int r = ty; r < 16; r += 4) {
+        int i = i0 + r;
+        if (i > N) break;
+        int li = isJ ? tx : r;
+        int k = isJ ? i : j;
+        int line = isJ ? j : i;
+        int first = 0, last = 0, n = s_moc[li];
+        bool found = false;
+        for (int s = 0; s < n; ++s) {
+            if (s_dau[li][s] <= k && k <= s_cuoi[li][s]) { first = s_dau[li][s]; last = s_cuoi[li][s]; found = true; break; }
+        }
+        if (!found || last < first + 2 || k < first + 1 || k > last - 1) continue;
+        bool br1, br2;
+        wall_flags2(d, isJ, line, first, last, br1, br2);
+        int pos = i * W + j;
+        double c = 18 * hd_log(12 * Hm[pos] / P.Ks);
+        double Uf = sqrt(P.g) * fabs(VTH[pos]) / c;
+        double wsm = P.wss * hd_pow((1 - FS[pos]), 4);
+        double Zf = 0, S = 0, gamav;
+        if (isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
+        if (FS[pos] > 0.00001) Zf = wsm / (0.4 * (Uf + 2 * wsm));
+        if (ketdinh) S = source_chs(P, wsm, Zf, FS[pos], Fw[pos], VTH[pos], Hm[pos]);
+        else S = source_nchs(P, wsm, Zf, Uf, Hm[pos], FS[pos]);
+        if (t < s_start) S = 0;
+        if (!isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
+        double A, B, C, D, p = 0, q = 0;
+        if (isJ) {
+            A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos - W] * Kx[pos - W] / (Hm[pos] * (P.dXbp));
+            C = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos] * Kx[pos] / (Hm[pos] * (P.dXbp));
+            B = P.HaiChiadT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (Hm[pos] * (P.dXbp));
+            if ((Hm[pos - 1] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) p = (FS[pos + 1] - FS[pos]);
+            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 1] > P.H_TINH)) q = (FS[pos] - FS[pos - 1]);
+            p = (1 / (Hm[pos] * P.dYbp)) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) {
+                q = (FS[pos + 1] - FS[pos - 1]) / P.dY2;
+                q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
+            } else q = 0;
+            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
+            if (i == first + 1) {
+                if ((br1) || (t_u[first * W + j] == 0)) B += A;
+                else D -= A * FS[first * W + j];
+            }
+            if (i == last - 1) {
+                if ((br2) && (t_u[last * W + j] == 0)) B += C;
+                else D -= C * FS[last * W + j];
+            }
+        } else {
+            A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos - 1] * Ky[pos - 1] / (Hm[pos] * P.dYbp);
+            C = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos] * Ky[pos] / (Hm[pos] * P.dYbp);
+            B = P.HaiChiadT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (Hm[pos] * P.dYbp);
+            if ((Hm[pos + 2 * W] > P.H_TINH) && (Hm[pos - W] > P.H_TINH)) p = (FS[pos + W] - FS[pos]);
+            if (Hm[pos - 2 * W] > P.H_TINH && Hm[pos + W] > P.H_TINH) q = (FS[pos] - FS[pos - W]);
+            p = (1 / (Hm[pos] * P.dXbp)) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+            if (Hm[pos + 2 * W] > P.H_TINH && Hm[pos - 2 * W] > P.H_TINH) {
+                q = (FS[pos + W] - FS[pos - W]) / P.dX2;
+                q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
+            } else q = 0;
+            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
+            if (j == first + 1) {
+                if ((br1) || (t_v[i * W + first] == 0)) B = B + A;
+                else D = D - A * FS[i * W + first];
+            }
+            if (j == last - 1) {
+                if ((br2) || (t_v[i * W + last] == 0)) B = B + C;
+                else D = D - C * FS[i * W + last];
+            }
+        }
+        size_t o = (size_t)line * tw + k;
+        d.AA[o] = A; d.BB[o] = B; d.CC[o] = C; d.DD[o] = D;
+    }
+}
+
+/* tridiagSolver (jump 1) + FSj/FSi_extract_solution (Sediment_Transport.cu:
+ * 305-338 / 168-202), one lane per line.  Runs with last - first < 2 have no
+ * rows and are skipped. */
+template <bool isJ>
+__global__ void __launch_bounds__(64) k_fs_solve(HdDev d)
+{
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int line = 3 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (line > (isJ ? M - 1 : N - 1)) return;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    int nseg = isJ ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    const int* dau = (isJ ? d.dauj : d.daui) + line * SL;
+    const int* cuoi = (isJ ? d.cuoij : d.cuoii) + line * SL;
+    size_t base = (size_t)line * tw;
+    const double *AA = d.AA + base, *BB = d.BB + base, *CC = d.CC + base, *DD = d.DD + base;
+    double *Ap = d.Ap + base, *Bp = d.Bp + base;
+    auto at = [&](int k) -> size_t { return isJ ? (size_t)k * W + line : (size_t)line * W + k; };
+    for (int s = 0; s < nseg; ++s) {
+        int first = dau[s], last = cuoi[s];
+        if (last < first + 2) continue;
+        bool br1, br2;
+        wall_flags2(d, isJ, line, first, last, br1, br2);
+        int sn = last - first - 2;
+        int o = first + 1;
+        double x;
+        if (sn == 0) x = DD[o] / BB[o];
+        else {
+            double ap = -CC[o] / BB[o], bp = DD[o] / BB[o];
+            Ap[o] = ap; Bp[o] = bp;
+            for (int r = 1; r < sn; ++r) {
+                double a = AA[o + r];
+                double e = a * ap + BB[o + r];
+                double nap = -CC[o + r] / e;
+                double nbp = (DD[o + r] - (a * bp)) / e;
+                ap = nap; bp = nbp;
+                Ap[o + r] = ap; Bp[o + r] = bp;
+            }
+            double a = AA[o + sn];
+            x = (DD[o + sn] - (a * bp)) / (BB[o + sn] + (a * ap));
+        }
+        double vfirst = 0, vlast = 0;
+        for (int r = sn; r >= 0; --r) {
+            if (r < sn) x = Bp[o + r] + (Ap[o + r] * x);
+            double v = x;
+            if (v < 0) v = d.P.NDnen;
+            d.tFS[at(o + r)] = v;
+            if (r == 0) vfirst = v;
+            if (r == sn) vlast = v;
+        }
+        /* end copies (FSj: t_u at first / last; FSi: t_v at first / last) */
+        const double* tq = isJ ? d.t_u : d.t_v;
+        if ((br1) || (tq[at(first)] == 0)) d.tFS[at(first)] = vfirst;
+        else d.tFS[at(first)] = d.FS[at(first)];
+        if ((br2) || (tq[at(last)] == 0)) d.tFS[at(last)] = vlast;
+        else d.tFS[at(last)] = d.FS[at(last)];
+    }
+}
+
+/* Update_FS, Sediment_Transport.cu:620-629. */
+__global__ void __launch_bounds__(256) k_update_fs(HdDev d)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int i = blockIdx.y + 1;
+    if (i > d.N || j > d.M) return;
+    d.FS[i * d.W + j] = d.tFS[i * d.W + j];
+}
+
+/* Calculate_Qb, Sediment_Transport.cu:341-414 (qb_mode REFERENCE keeps the
+ * `!VTH` test as written, FIXED uses VTH != 0; corner copies braced). */
+__global__ void __launch_bounds__(256) k_calc_qb(HdDev d, int qb_mode)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    int i = blockIdx.y + 2;
+    int N = d.N, M = d.M, W = d.W;
+    if (i > N || j > M) return;
+    const hd_params& P = d.P;
+    double* Qbx = d.Qbx;
+    double* Qby = d.Qby;
+    int pos = i * W + j;
+    double Tob, Toee = P.Toe, Tx = 0;
+    double vth = d.VTH[pos];
+    bool sel = (qb_mode == HD_QB_REFERENCE) ? (!vth) : (vth != 0);
+    if (sel && (d.khouot[pos] == 0)) {
+        Tob = P.ro * d.Fw[pos] * (vth * vth);
+        if (d.H_moi[pos] > P.ghtoe) Toee = P.Toe * (1 + P.hstoe * ((d.H_moi[pos] - P.ghtoe)));
+        if (Tob > Toee) Tx = Tob / Toee - 1;
+        Qbx[pos] = 0.053 * hd_pow((P.Sx - 1) * P.g, 0.5) * hd_pow(P.dm, 1.5) * hd_pow(Tx, 2.1) * hd_pow(P.Dxr, -0.3)
+                   * (d.t_u[pos] + d.t_u[pos - W]) * 0.5 / vth;
+        Qby[pos] = 0.053 * hd_pow((P.Sx - 1) * P.g, 0.5) * hd_pow(P.dm, 1.5) * hd_pow(Tx, 2.1) * hd_pow(P.Dxr, -0.3)
+                   * (d.t_v[pos] + d.t_v[pos - 1]) * 0.5 / vth;
+    } else {
+        Qbx[pos] = 0;
+        Qby[pos] = 0;
+    }
+    double qx = Qbx[pos], qy = Qby[pos];
+    if (i == 2) { Qbx[pos - W] = qx; Qby[pos - W] = qy; }
+    if (i == N) { Qbx[pos + W] = qx; Qby[pos + W] = qy; }
+    if (j == 2) {
+        Qbx[pos - 1] = qx; Qby[pos - 1] = qy;
+        if (i == 2) { Qbx[1 * W + 1] = qx; Qby[1 * W + 1] = qy; }
+        if (i == N) { Qbx[(N + 1) * W + 1] = qx; Qby[(N + 1) * W + 1] = qy; }
+    }
+    if (j == M) {
+        Qbx[pos + 1] = qx; Qby[pos + 1] = qy;
+        if (i == 2) { Qbx[1 * W + M + 1] = qx; Qby[1 * W + M + 1] = qy; }
+        if (i == N) { Qbx[(N + 1) * W + M + 1] = qx; Qby[(N + 1) * W + M + 1] = qy; }
+    }
+}
+
+/* _bed_load via BedLoad, Sediment_Transport.cu:416-506 / 630-639: rows
+ * [sidx, eidx-1], cells of row runs with last - first >= 2 (t_u[pos > 0]
+ * reads t_u[1], as written). */
+__global__ void __launch_bounds__(256) k_bedload(HdDev d, int ketdinh, int sidx, int eidx)
+{
+    __shared__ int s_moc;
+    __shared__ int s_dau[24], s_cuoi[24];
+    int i = sidx + blockIdx.y;
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    if (i >= eidx) return;
+    if (threadIdx.x == 0) s_moc = min(d.moci[i], SL);
+    __syncthreads();
+    if (threadIdx.x < s_moc) { s_dau[threadIdx.x] = d.daui[i * SL + threadIdx.x]; s_cuoi[threadIdx.x] = d.cuoii[i * SL + threadIdx.x]; }
+    __syncthreads();
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    if (j > M) return;
+    int first = 0, last = 0;
+    bool found = false;
+    for (int s = 0; s < s_moc; ++s)
+        if (s_dau[s] <= j && j <= s_cuoi[s]) { first = s_dau[s]; last = s_cuoi[s]; found = true; break; }
+    if (!found || last - first < 2) return;
+    const hd_params& P = d.P;
+    const double *FS = d.FS, *Hm = d.H_moi, *t_u = d.t_u, *t_v = d.t_v, *VTH = d.VTH, *Htdu = d.Htdu, *Htdv = d.Htdv,
+                 *Fw = d.Fw, *Kx = d.Kx, *Ky = d.Ky, *Qbx = d.Qbx, *Qby = d.Qby;
+    const int* kh = d.khouot;
+    int pos = i * W + j;
+    double p = 0, q = 0;
+    if (kh[pos - W] == 1) {
+        if (t_u[pos] < 0) p = (-3 * Qbx[pos] + 4 * Qbx[pos + W] - Qbx[pos + 2 * W]) / P.dX2;
+    } else {
+        if (kh[pos + W] == 1) {
+            if (t_u[1]) p = (3 * Qbx[pos] - 4 * Qbx[pos - W] + Qbx[pos - 2 * W]) / P.dX2;
+        } else p = (Qbx[pos + W] - Qbx[pos - W]) / P.dX2;
+    }
+    if (kh[pos - 1] == 1) {
+        if (t_v[pos] < 0) q = (-3 * Qby[pos] + 4 * Qby[pos + 1] - Qby[pos + 2]) / P.dY2;
+    } else {
+        if (kh[pos + 1] == 1) {
+            if (t_v[pos] > 0) q = (3 * Qby[pos] - 4 * Qby[pos - 1] + Qby[pos - 2]) / P.dY2;
+        } else q = (Qby[pos + 1] - Qby[pos - 1]) / P.dY2;
+    }
+    double tH = p + q;
+    p = FS[pos + W] - FS[pos];
+    if ((kh[pos + 2 * W] == 1 || kh[pos - W] == 1) && (i < N)) p = 0;
+    q = FS[pos] - FS[pos - W];
+    if ((i > 2) && (kh[pos - 2 * W] == 1 || kh[pos + W] == 1)) q = 0;
+    p = 1 / P.dXbp * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+    tH = tH + p;
+    p = FS[pos + 1] - FS[pos];
+    if (((last < M) && (kh[pos - 1] == 1 || kh[pos + 2] == 1)) || (last == M && (kh[pos - 1] == 1))) p = 0;
+    q = FS[pos] - FS[pos - 1];
+    if (((first > 2) && (kh[pos - 2] == 1 || kh[pos + 1] == 1)) || (first == 2 && kh[pos + 1] == 1)) q = 0;
+    p = 1 / (P.dYbp) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+    tH = tH + p;
+    double c = 18 * hd_log(12 * Hm[pos] / P.Ks);
+    double Uf = sqrt(P.g) * fabs(VTH[pos]) / c;
+    double wsm = P.wss * hd_pow((1 - FS[pos]), 4);
+    double Zf = 0, S = 0;
+    if (FS[pos] > 0.0) Zf = wsm / (0.4 * (Uf + 2 * wsm));
+    if (ketdinh) S = source_chs(P, wsm, Zf, FS[pos], Fw[pos], VTH[pos], Hm[pos]);
+    else S = source_nchs(P, wsm, Zf, Uf, Hm[pos], FS[pos]);
+    d.dH[pos] += P.dT / (1 - P.Dorong) * (S + tH);
+}
+
+} // namespace
+
+void hd_launch_vth_hesok(const HdDev& d, hipStream_t s)
+{
+    dim3 g2((d.M + 1 + 255) / 256, d.N + 1);
+    hipLaunchKernelGGL(k_vth_hesok, g2, dim3(256), 0, s, d);
+}
+
+void hd_launch_fs_coef(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s)
+{
+    dim3 blk(64, 4);
+    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
+    if (which == 0) hipLaunchKernelGGL(k_fs_coef<true>, grd, blk, 0, s, d, tdev, s_start, ketdinh);
+    else hipLaunchKernelGGL(k_fs_coef<false>, grd, blk, 0, s, d, tdev, s_start, ketdinh);
+}
+
+void hd_launch_fs_solve(const HdDev& d, int which, hipStream_t s)
+{
+    int nl = (which == 0 ? d.M : d.N) - 3;
+    if (nl <= 0) return;
+    if (which == 0) hipLaunchKernelGGL(k_fs_solve<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d);
+    else hipLaunchKernelGGL(k_fs_solve<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d);
+}
+
+void hd_launch_update_fs(const HdDev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_fs, dim3((d.M + 255) / 256, d.N), dim3(256), 0, s, d);
+}
+
+void hd_launch_calc_qb(const HdDev& d, int qb_mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_calc_qb, dim3((d.M - 1 + 255) / 256, d.N - 1), dim3(256), 0, s, d, qb_mode);
+}
+
+void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStream_t s)
+{
+    int nr = eidx - sidx;
+    if (nr > 0) hipLaunchKernelGGL(k_bedload, dim3((d.M - 1 + 255) / 256, nr), dim3(256), 0, s, d, ketdinh, sidx, eidx);
+}

@@ -1,0 +1,229 @@
+"""ctypes binding of libhydist.so (include/hydist.h).
+
+The library is built in-tree (hydist_amd/_build/libhydist.so) by
+``hydist_amd.build.build()``.  There is no fallback: if the shared library is
+missing or no HIP device is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .case import HdParams, HdStepOpts
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libhydist.so")
+
+# every entry point declared in include/hydist.h
+EXPORTS = (
+    "hd_version", "hd_error_string", "hd_create", "hd_destroy", "hd_last_error", "hd_get_params",
+    "hd_field_info", "hd_upload", "hd_download", "hd_get_hmax", "hd_init", "hd_prepare", "hd_step", "hd_run",
+    "hd_halfstep", "hd_bed_change", "hd_run_counted", "hd_reset_wet_counter", "hd_wet_cell_steps",
+    "hd_wet_count", "hd_sync", "hd_struct_ptrs", "hd_profile", "hd_kernel_times", "hd_thomas_batch",
+)
+
+_lib = None
+
+
+class HydistError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libhydist error {code}: {msg}")
+        self.code = code
+
+
+def load() -> ctypes.CDLL:
+    """Load libhydist.so and declare its signatures (no device call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is not built; run hydist_amd.build.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, cp, sz, ci, cd = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+    pp, po = ctypes.POINTER(HdParams), ctypes.POINTER(HdStepOpts)
+    dp, ip, ll = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)
+    sig = {
+        "hd_version": (cp, []),
+        "hd_error_string": (cp, [ci]),
+        "hd_create": (ci, [ci, pp, ci, ci, ci, ci, ctypes.POINTER(vp)]),
+        "hd_destroy": (ci, [vp]),
+        "hd_last_error": (cp, [vp]),
+        "hd_get_params": (ci, [vp, pp]),
+        "hd_field_info": (ci, [vp, cp, ctypes.POINTER(sz), ip]),
+        "hd_upload": (ci, [vp, cp, vp, sz]),
+        "hd_download": (ci, [vp, cp, vp, sz]),
+        "hd_get_hmax": (ci, [vp, dp]),
+        "hd_init": (ci, [vp]),
+        "hd_prepare": (ci, [vp]),
+        "hd_step": (ci, [vp, cd, po, dp]),
+        "hd_run": (ci, [vp, cd, ci, po, dp]),
+        "hd_halfstep": (ci, [vp, ci, cd, po]),
+        "hd_bed_change": (ci, [vp, cd, po]),
+        "hd_run_counted": (ci, [vp, cd, ci, po, dp]),
+        "hd_reset_wet_counter": (ci, [vp]),
+        "hd_wet_cell_steps": (ci, [vp, ll]),
+        "hd_wet_count": (ci, [vp, ll]),
+        "hd_sync": (ci, [vp]),
+        "hd_struct_ptrs": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "hd_profile": (ci, [vp, ci]),
+        "hd_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
+        "hd_thomas_batch": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None):
+    if rc != 0:
+        L = load()
+        msg = L.hd_last_error(ctx).decode()
+        raise HydistError(rc, msg or L.hd_error_string(rc).decode())
+
+
+class Solver:
+    """One case on one GPU: the device state of the reference's Pointers
+    (rep.py:62-111) plus the step of Engine.hydraulic_Calculation."""
+
+    def __init__(self, case=None, *, params: HdParams = None, N: int = None, M: int = None,
+                 series_len: int = None, total_time: int = None, device: int = 0, arrays=None):
+        L = load()
+        if case is not None:
+            params, N, M = case.params, case.N, case.M
+            series_len, total_time = case.series_len, case.total_time
+            arrays = case.arrays if arrays is None else arrays
+        self.N, self.M = N, M
+        self._params = params
+        self.h = ctypes.c_void_p()
+        check(L.hd_create(device, ctypes.byref(params), N, M, series_len, total_time, ctypes.byref(self.h)))
+        self.case = case
+        if arrays:
+            for k, a in arrays.items():
+                self.set(k, a)
+
+    def close(self):
+        if self.h:
+            load().hd_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        check(rc, self.h)
+
+    def field_info(self, name):
+        nb, dt = ctypes.c_size_t(), ctypes.c_int()
+        self._chk(load().hd_field_info(self.h, name.encode(), ctypes.byref(nb), ctypes.byref(dt)))
+        return nb.value, (np.float64 if dt.value == 0 else np.int32)
+
+    def set(self, name, arr):
+        nb, dt = self.field_info(name)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        self._chk(load().hd_upload(self.h, name.encode(), a.ctypes.data, a.nbytes))
+
+    def get(self, name, like=None):
+        nb, dt = self.field_info(name)
+        if like is not None:
+            out = np.empty(like.shape, dtype=dt)
+        elif self.case is not None and name in self.case.arrays:
+            out = np.empty(self.case.arrays[name].shape, dtype=dt)
+        else:
+            out = np.empty(nb // np.dtype(dt).itemsize, dtype=dt)
+            if out.size == (self.N + 3) * (self.M + 3):
+                out = out.reshape(self.N + 3, self.M + 3)
+        self._chk(load().hd_download(self.h, name.encode(), out.ctypes.data, out.nbytes))
+        return out
+
+    def params(self) -> HdParams:
+        p = HdParams()
+        self._chk(load().hd_get_params(self.h, ctypes.byref(p)))
+        return p
+
+    def hmax(self):
+        out = (ctypes.c_double * 4)()
+        self._chk(load().hd_get_hmax(self.h, out))
+        return list(out)
+
+    def init(self):
+        self._chk(load().hd_init(self.h))
+
+    def prepare(self):
+        self._chk(load().hd_prepare(self.h))
+
+    def step(self, t, opts):
+        tout = ctypes.c_double()
+        self._chk(load().hd_step(self.h, t, ctypes.byref(opts), ctypes.byref(tout)))
+        return tout.value
+
+    def run(self, t, nsteps, opts, counted=False):
+        tout = ctypes.c_double()
+        fn = load().hd_run_counted if counted else load().hd_run
+        self._chk(fn(self.h, t, nsteps, ctypes.byref(opts), ctypes.byref(tout)))
+        return tout.value
+
+    def halfstep(self, which, t, opts):
+        self._chk(load().hd_halfstep(self.h, which, t, ctypes.byref(opts)))
+
+    def bed_change(self, t, opts):
+        self._chk(load().hd_bed_change(self.h, t, ctypes.byref(opts)))
+
+    def sync(self):
+        self._chk(load().hd_sync(self.h))
+
+    def wet_count(self):
+        n = ctypes.c_longlong()
+        self._chk(load().hd_wet_count(self.h, ctypes.byref(n)))
+        return n.value
+
+    def reset_wet_counter(self):
+        self._chk(load().hd_reset_wet_counter(self.h))
+
+    def wet_cell_steps(self):
+        n = ctypes.c_longlong()
+        self._chk(load().hd_wet_cell_steps(self.h, ctypes.byref(n)))
+        return n.value
+
+    def profile(self, enable=True):
+        self._chk(load().hd_profile(self.h, int(enable)))
+
+    def kernel_times(self, n=64):
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        k = load().hd_kernel_times(self.h, n, names, ms, cnt)
+        if k < 0:
+            self._chk(k)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def struct_ptrs(self):
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        self._chk(load().hd_struct_ptrs(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+def thomas_batch(systems, device=0):
+    """Solve a list of (A, B, C, D) tridiagonal systems with the GPU Thomas
+    kernel; returns the list of solutions."""
+    L = load()
+    off, sn, rows = [], [], 0
+    for A, *_ in systems:
+        off.append(rows)
+        sn.append(len(A) - 1)
+        rows += len(A)
+    cat = [np.ascontiguousarray(np.concatenate([s[k] for s in systems]), dtype=np.float64) for k in range(4)]
+    x = np.zeros(rows)
+    off_a = np.asarray(off, np.int32)
+    sn_a = np.asarray(sn, np.int32)
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    check(L.hd_thomas_batch(device, len(systems), off_a.ctypes.data_as(ip), sn_a.ctypes.data_as(ip),
+                            *(c.ctypes.data_as(dp) for c in cat), x.ctypes.data_as(dp), rows))
+    return [x[o:o + n + 1] for o, n in zip(off, sn)]

@@ -1,0 +1,181 @@
+/*
+ * hydist.h -- C ABI of libhydist.so, the MI355X (gfx950) FP64 ADI time step of
+ * the HYDIST 2-D shallow-water and suspended-sediment model.
+ *
+ * The reference drives its CUDA kernels from Python through PyCUDA:
+ * SourceModule(...).get_function(name) and one launch per kernel
+ * (Engine.py:73-128, Engine.py:144-366, rep.py:115-139), with arrays owned by
+ * Pointers (Pointers.py:15-166) and packed into the device structs
+ * Argument_Pointers / Array_Pointers (constant.cuh:6-29, Pointers.py:169-191).
+ * This library replaces that boundary: the Python host (hydist_amd package)
+ * binds these functions with ctypes.  No torch or HIP types cross it: plain
+ * pointers, sizes and the two POD structs below.
+ *
+ * Conventions (identical to the reference's, Global_Variables.py:68-127):
+ *  - N rows (index i, x, u, UZ lines run along i), M columns (index j, y, v,
+ *    VZ lines run along j); every 2-D field is (N+3) x (M+3) row-major,
+ *    flat index i*(M+3)+j, FP64 except the int32 mask khouot.
+ *  - Segment tables daui/cuoii [(N+2) x SL], dauj/cuoij [(M+2) x SL],
+ *    moci [N+2], mocj [M+2] (int32), SL = segment_limit.
+ *  - Boundary series bc_* / CC_* are laid out seg*total_time + hour.
+ *
+ * Ownership: the library owns all device memory; the caller owns host buffers,
+ * which are copied synchronously by hd_upload / hd_download.  Errors: every
+ * function returns HD_OK (0) or a negative HD_E* code and never aborts;
+ * hd_last_error() gives the message.  Threading: one hd_ctx per host thread;
+ * all work is queued on the context's own HIP stream.
+ */
+#ifndef HYDIST_H
+#define HYDIST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HD_OK 0
+#define HD_EINVAL (-1)        /* bad argument */
+#define HD_ESHAPE (-2)        /* byte count does not match the field layout */
+#define HD_ENAME (-3)         /* unknown field name */
+#define HD_EHIP (-4)          /* HIP runtime error */
+#define HD_ESEGOVERFLOW (-5)  /* a line has more wet runs than segment_limit */
+#define HD_ESTATE (-6)        /* call out of order (e.g. step before init) */
+#define HD_ERANGE (-7)        /* boundary series too short for the time t */
+#define HD_ENOMEM (-8)
+
+/* Case parameters: the __constant__ values of a constant.cuh case header
+ * (constant.cuh:31-118, PI from constant.cuh:3), evaluated in IEEE double
+ * exactly as written there.  The last five members are the header's constant
+ * device functions Windx/Windy/Ufr/Dxr/wss (constant.cuh:126-133); hd_create
+ * evaluates them from the other members and ignores the values passed in. */
+typedef struct hd_params {
+    int32_t segment_limit;
+    int32_t pad0;
+    double dX, dY, dT;
+    double dXbp, dX2, dYbp, dY2;
+    double dTchia2dX, dTchia2dY;
+    double QuyDoiTime, QuyDoiPi, HaiChiadT;
+    double kenhhepng, kenhhepd;
+    double NANGDAY, H_TINH;
+    double Wind, huonggio;
+    double Zbandau, heso, mu_mn;
+    double NDnen, NDbphai, NDbtrai, NDbtren, NDbduoi;
+    double Tod, Toe, hstoe, ghtoe, Mbochat;
+    double ro, ros, dm, d90, muy, Dorong, Sx;
+    double sohn, soha, sohn1;
+    double CORIOLIS_FORCE, g, Ks;
+    double PI;
+    double Windx, Windy, Ufr, Dxr, wss;
+} hd_params;
+
+/* Boundary handling, Engine.py:29-46: the reference forces the analytic tide
+ * (boundary_up/down/left/right, support_funcs.cu:356-429) whenever a narrow
+ * channel flag is set, and the hourly series (Update_Boundary_Value,
+ * support_funcs.cu:622-668) otherwise. */
+#define HD_BND_AUTO 0
+#define HD_BND_SERIES 1
+#define HD_BND_ANALYTIC 2
+
+/* Bed-load flux rule: REFERENCE computes Qb only where VTH == 0, as
+ * Sediment_Transport.cu:365 is written; FIXED computes it where VTH != 0. */
+#define HD_QB_REFERENCE 0
+#define HD_QB_FIXED 1
+
+/* Bed-change trigger: PERIODIC fires when int(t) % bed_change_start == 0 at an
+ * integral t (Engine.py:364); EVERY_STEP fires at every step with
+ * t >= bed_change_start. */
+#define HD_BED_PERIODIC 0
+#define HD_BED_EVERY_STEP 1
+
+/* Keyword options of hydraulic_Calculation (Engine.py:49-50) that change the
+ * step, plus the explicit modes above. */
+typedef struct hd_step_opts {
+    int32_t kenhhepd;
+    int32_t kenhhepng;
+    int32_t boundary_mode;
+    int32_t ketdinh;
+    int32_t qb_mode;
+    int32_t bed_mode;
+    double sediment_start;
+    double bed_change_start;
+} hd_step_opts;
+
+typedef struct hd_ctx hd_ctx;
+
+/* Library / context */
+const char* hd_version(void);
+const char* hd_error_string(int code);
+/* Create a context on HIP device `device` for an N x M case.  series_len is
+ * the length (doubles) of each boundary series bc_* and CC_*; total_time is
+ * the number of hourly samples per segment (Global_Variables.py:53). */
+int hd_create(int device, const hd_params* p, int N, int M, int series_len,
+              int total_time, hd_ctx** out);
+int hd_destroy(hd_ctx* c);
+const char* hd_last_error(const hd_ctx* c);
+/* Evaluated case parameters (derived members filled in). */
+int hd_get_params(const hd_ctx* c, hd_params* out);
+
+/* Named fields: the Pointers keys of rep.py:62-70 (VISCOINDX is accepted as an
+ * alias of VISCOIDX).  dtype: 0 = float64, 1 = int32. */
+int hd_field_info(hd_ctx* c, const char* name, size_t* nbytes, int* dtype);
+int hd_upload(hd_ctx* c, const char* name, const void* src, size_t nbytes);
+int hd_download(hd_ctx* c, const char* name, void* dst, size_t nbytes);
+/* hmax_u, hmax_d, hmax_l, hmax_r (Argument_Pointers, constant.cuh:8). */
+int hd_get_hmax(hd_ctx* c, double out[4]);
+
+/* One-time initialisation, rep.py:126-139: Onetime_init, the segment finders,
+ * Htuongdoi and preprocess_data.  Call after uploading the input fields. */
+int hd_init(hd_ctx* c);
+/* rep.py:135-139 alone (segment finders, Htuongdoi, preprocess_data) on the
+ * current state: after uploading a pick-up state (rep.py:43-55, 133-134). */
+int hd_prepare(hd_ctx* c);
+
+/* One full ADI step t -> t + dT (both half-steps, sediment and bed change as
+ * the options select), Engine.py:144-366.  *t_out receives the new time. */
+int hd_step(hd_ctx* c, double t, const hd_step_opts* o, double* t_out);
+/* nsteps consecutive steps, replayed from a captured HIP graph when the
+ * sequence is graph-capturable; identical results to nsteps hd_step calls. */
+int hd_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out);
+/* Stages of a step, in Engine.py order, for drivers that interleave host work.
+ * which: 0 = UZ half-step (first half), 1 = VZ half-step (second half).
+ * t is the time after the half-step's t += dT/2. */
+int hd_halfstep(hd_ctx* c, int which, double t, const hd_step_opts* o);
+int hd_bed_change(hd_ctx* c, double t, const hd_step_opts* o);
+
+/* hd_run that also adds, after every step, the number of wet cells
+ * (khouot == 0 over the padded grid) of the step's end state to a device
+ * counter: the wet-cell-steps of the run's steps 2..n+1 start states. */
+int hd_run_counted(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out);
+int hd_reset_wet_counter(hd_ctx* c);
+int hd_wet_cell_steps(hd_ctx* c, long long* out);
+
+/* Number of wet cells (khouot == 0) over the padded grid. */
+int hd_wet_count(hd_ctx* c, long long* out);
+int hd_sync(hd_ctx* c);
+/* Device addresses of the Argument_Pointers / Array_Pointers mirrors
+ * (constant.cuh:6-29), for callers that hand the structs to their own code. */
+int hd_struct_ptrs(hd_ctx* c, void** arg, void** arr);
+/* Per-kernel timing with HIP events recorded on the context's stream around
+ * every launch.  hd_profile(c, 1) clears the accumulators and turns it on
+ * (hd_run then enqueues steps directly instead of replaying graphs);
+ * hd_kernel_times fills up to n (name, total ms, launch count) entries in
+ * first-launch order and returns how many it filled. */
+int hd_profile(hd_ctx* c, int enable);
+int hd_kernel_times(hd_ctx* c, int n, const char** names, double* total_ms, int* launches);
+
+/* Stand-alone kernels for unit parity. */
+/* Batched Thomas solve (UVZSolver_multithread.cu:203-228 applied per system):
+ * nsys systems, system s occupies rows off[s] .. off[s]+sn[s] of the device
+ * arrays A,B,C,D (lower, diagonal, upper, rhs); x receives the solution.
+ * All pointers are host pointers; the call copies in and out. */
+int hd_thomas_batch(int device, int nsys, const int* off, const int* sn,
+                    const double* A, const double* B, const double* C,
+                    const double* D, double* x, int nrows);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDIST_H */

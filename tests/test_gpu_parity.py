@@ -1,0 +1,144 @@
+"""GPU parity: libhydist (hand-written gfx950 kernels, through the C ABI)
+against the CPU checker, bit for bit, on seeded synthetic cases small enough
+for the checker to finish in seconds; plus error paths and run == step."""
+import numpy as np
+import pytest
+
+import oracle
+from hydist_amd import synth
+from hydist_amd._lib import HydistError, Solver, thomas_batch
+from hydist_amd.case import BED_EVERY_STEP, QB_REFERENCE
+
+from _parity import assert_same_state
+
+pytestmark = pytest.mark.gpu
+
+
+def small_channel():
+    return synth.channel_case(N=40, M=12, nsteps=60, name="channel_40x12")
+
+
+def small_river():
+    return synth.config(2, scale=0.1)
+
+
+def small_river_sediment(qb_mode=None):
+    c = synth.config(3, scale=0.1)
+    c.sediment_start = 0.0
+    c.bed_change_start = 10.0
+    c.bed_mode = BED_EVERY_STEP
+    if qb_mode is not None:
+        c.qb_mode = qb_mode
+    return c
+
+
+def pair(case):
+    o = oracle.Oracle(case)
+    g = Solver(case)
+    o.init()
+    g.init()
+    return o, g
+
+
+def test_thomas_batch_bitwise():
+    rng = np.random.default_rng(0)
+    systems = []
+    for n in (1, 2, 3, 7, 64, 333, 1961):
+        A = rng.normal(size=n); C = rng.normal(size=n)
+        B = 3 + np.abs(A) + np.abs(C); D = rng.normal(size=n)
+        systems.append((A, B, C, D))
+    for x, s in zip(thomas_batch(systems), systems):
+        assert np.array_equal(x, oracle.thomas(*s))
+
+
+@pytest.mark.parametrize("make", [small_channel, small_river, small_river_sediment])
+def test_init_parity(make):
+    case = make()
+    o, g = pair(case)
+    assert_same_state(o, g, "after init")
+    assert o.hmax() == g.hmax()
+
+
+@pytest.mark.parametrize("make,nsteps", [(small_channel, 40), (small_river, 40), (small_river_sediment, 30)])
+def test_halfsteps_and_steps_bitwise(make, nsteps):
+    case = make()
+    o, g = pair(case)
+    op = case.opts()
+    t = 0.0
+    for which in (0, 1):
+        t = t + case.dT * 0.5
+        o.halfstep(which, t, op)
+        g.halfstep(which, t, op)
+        assert_same_state(o, g, f"half-step {which}")
+    for k in range(nsteps):
+        tn = o.step(t, op)
+        g.step(t, op)
+        t = tn
+        if k % 10 == 9 or k == nsteps - 1:
+            assert_same_state(o, g, f"step {k + 2}")
+
+
+def test_bed_change_reference_mode():
+    case = small_river_sediment(qb_mode=QB_REFERENCE)
+    o, g = pair(case)
+    op = case.opts()
+    t = o.run(0.0, 6, op)
+    g.run(0.0, 6, op)
+    assert_same_state(o, g, "reference Qb mode")
+
+
+def test_run_equals_step():
+    case = small_river()
+    g1, g2 = Solver(case), Solver(case)
+    g1.init()
+    g2.init()
+    op = case.opts()
+    t = 0.0
+    for _ in range(25):
+        t = g1.step(t, op)
+    t2 = g2.run(0.0, 25, op)
+    assert t == t2
+    assert_same_state(g1, g2, "graph replay vs step")
+
+
+def overflow_case():
+    """A channel whose rows are cut by three dry walls into four wet runs,
+    with segment_limit = 3."""
+    case = synth.channel_case(N=30, M=40, nsteps=10, SL=3, name="overflow")
+    h = case.arrays["h"].copy()
+    for k in (10, 18, 26):
+        h[:, k:k + 2] = 0.0
+    case.arrays["h"] = synth.remove_pinches(h)
+    case.arrays["boundary_type"] = np.zeros((4, 3), np.int32)
+    case.arrays["bienQ"][:] = 0
+    return case
+
+
+def test_segment_overflow_raises():
+    case = overflow_case()
+    o = oracle.Oracle(case)
+    with pytest.raises(oracle.OracleError):
+        o.init()
+    g = Solver(case)
+    with pytest.raises(HydistError) as e:
+        g.init()
+    assert e.value.code == -5
+
+
+def test_boundary_series_range_error():
+    case = small_channel()
+    g = Solver(case)
+    g.init()
+    with pytest.raises(HydistError) as e:
+        g.run(case.total_time * 3600.0, 1, case.opts())
+    assert e.value.code == -7
+
+
+def test_analytic_channel_boundaries():
+    case = small_channel()
+    case.boundary_mode = 0   # the reference rule: channel flag -> analytic tide
+    o, g = pair(case)
+    op = case.opts()
+    o.run(0.0, 10, op)
+    g.run(0.0, 10, op)
+    assert_same_state(o, g, "analytic channel mode")
