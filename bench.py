@@ -29,10 +29,13 @@ B_STEP = 256               # algorithmic bytes per wet-cell-step (hydrodynamics)
 # Algorithmic bytes per wet cell of the per-launch kernels (compulsory FP64
 # reads/writes of the kernel's inputs and outputs, each element counted once).
 KERNEL_BYTES = {
-    # rows A,B,C,D of two rows read, solution (z and u) written
+    # fused assembly + Thomas + extraction of one half-step: u, v, z, Htdu,
+    # Htdv, H_moi, Kx1|Ky1, VISCOIDX, Tsxw|Tsyw read, t_z and t_u|t_v written
+    "adi_line_uz": 9 * 8 + 2 * 8,
+    "adi_line_vz": 9 * 8 + 2 * 8,
+    # two-kernel form (HYDIST_SOLVER=rows): rows of A,B,C,D round-trip HBM
     "adi_solve_uz": 2 * 4 * 8 + 2 * 8,
     "adi_solve_vz": 2 * 4 * 8 + 2 * 8,
-    # u,v,z,Htdu,Htdv,H_moi,Kx1|Ky1,VISCOIDX,Tsxw|Tsyw read, two rows written
     "adi_coef_uz": 9 * 8 + 2 * 4 * 8,
     "adi_coef_vz": 9 * 8 + 2 * 4 * 8,
 }
@@ -44,27 +47,43 @@ def parse():
     p.add_argument("--steps", type=int, default=720)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--profile-steps", type=int, default=20)
-    p.add_argument("--cpu-steps", type=int, default=100)
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
     return p.parse_args()
 
 
-def cpu_baseline(case, nsteps):
-    """The CPU checker (same FP64 method, one thread) on the first `nsteps`
-    steps of the same run."""
+def cpu_threads() -> int:
+    """Host threads for the CPU leg: the cores this process may run on,
+    at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(case, budget_s, threads):
+    """The CPU checker (same FP64 method, OpenMP over independent lines) on
+    the steps of the same run from t = 0 that fit in about `budget_s`
+    seconds; returns (wet-cell-steps/s, steps, seconds)."""
     import oracle
+    used = oracle.set_threads(threads)
     o = oracle.Oracle(case)
     o.init()
     opts = case.opts()
     wet = 0
     t = 0.0
+    n = 0
     t0 = time.perf_counter()
-    for _ in range(nsteps):
+    while True:
         wet += o.wet_count()
         t = o.step(t, opts)
-    dt = time.perf_counter() - t0
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or n >= 100000:
+            break
     o.close()
-    return wet / dt, dt
+    return wet / dt, n, dt, used
 
 
 def main():
@@ -89,8 +108,9 @@ def main():
     t = g.run(t, a.steps, opts, counted=True)
     g.sync()
     elapsed = time.perf_counter() - t0
+    acc = g.wet_cell_steps()                     # end states of the K timed steps
     wc1 = g.wet_count()
-    wet_steps = g.wet_cell_steps() + wc0 - wc1   # start states of the K timed steps
+    wet_steps = acc + wc0 - wc1                  # start states of the K timed steps
     value = wet_steps / elapsed
     ms_per_step = elapsed * 1e3 / a.steps
 
@@ -122,10 +142,10 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        rate, dt = cpu_baseline(case, a.cpu_steps)
-        cpu = {"value": rate, "unit": "wet-cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"first {a.cpu_steps} steps of river_980x1048 from t=0, CPU checker, 1 thread "
-                         f"({dt:.1f} s)"}
+        rate, n, dt, used = cpu_baseline(case, a.cpu_seconds, cpu_threads())
+        cpu = {"value": rate, "unit": "wet-cell-steps/s", "cores": used, "kind": "port",
+               "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
+                         f"(FP64 restatement, OpenMP over lines, {used} threads)"}
 
     line = {
         "metric": "ADI wet-cell-steps/s (FP64)",

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -83,6 +84,7 @@ struct hd_ctx {
     bool initialised = false;
     std::string err;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    bool solver_v1 = false;     /* two-kernel rows + lane-per-line solve (HYDIST_SOLVER=rows) */
     /* per-kernel HIP-event timing (hd_profile / hd_kernel_times) */
     bool profile = false;
     struct Rec { const char* name; hipEvent_t a, b; };
@@ -190,6 +192,10 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     c->N = N; c->M = M; c->SL = p->segment_limit; c->series_len = series_len; c->total_time = total_time;
     c->P = *p;
     derive_params(c->P);
+    {
+        const char* sv = getenv("HYDIST_SOLVER");
+        c->solver_v1 = sv && std::string(sv) == "rows";
+    }
     int rc;
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, HD_EHIP, "hipSetDevice failed"); }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -243,7 +249,8 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     if ((rc = dalloc(c, (void**)&d.seg_end, nlines * SL * 6 * 8))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.seg_sn, nlines * SL * 4))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.flags, 16 * 4))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, (void**)&d.wet_count, 8))) { hd_destroy(c); return rc; }
+    /* [0]: per-step accumulator of hd_run_counted, [1]: one-shot count of hd_wet_count */
+    if ((rc = dalloc(c, (void**)&d.wet_count, 16))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&c->tdev, 8))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&c->cand_cnt, nlines * 4))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&c->cand, nlines * 64 * 4))) { hd_destroy(c); return rc; }
@@ -426,8 +433,12 @@ static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sedim
     if (which == 0) {
         sidx = 2; eidx = M;
         if (channel && o.kenhhepd == 1) { sidx = 3; eidx = M - 1; }
-        launch(c, "adi_coef_uz", [&] { hd_launch_adi_coef(c->d, 1, sidx, eidx, s); });
-        launch(c, "adi_solve_uz", [&] { hd_launch_adi_solve(c->d, 1, sidx, eidx, s); });
+        if (c->solver_v1) {
+            launch(c, "adi_coef_uz", [&] { hd_launch_adi_coef(c->d, 1, sidx, eidx, s); });
+            launch(c, "adi_solve_uz", [&] { hd_launch_adi_solve(c->d, 1, sidx, eidx, s); });
+        } else {
+            launch(c, "adi_line_uz", [&] { hd_launch_adi_line(c->d, 1, sidx, eidx, s); });
+        }
         launch(c, "smooth_u", [&] { hd_launch_normalize_post(c->d, 1, 0, 1, c->cand_cnt, c->cand, c->dryflag, s); });
         swap_u(c);
         launch(c, "explicit_v", [&] { hd_launch_explicit(c->d, 0, 2, N, s); });
@@ -438,8 +449,12 @@ static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sedim
     } else {
         sidx = 2; eidx = N;
         if (channel && o.kenhhepng == 1) { sidx = 3; eidx = N; }
-        launch(c, "adi_coef_vz", [&] { hd_launch_adi_coef(c->d, 0, sidx, eidx, s); });
-        launch(c, "adi_solve_vz", [&] { hd_launch_adi_solve(c->d, 0, sidx, eidx, s); });
+        if (c->solver_v1) {
+            launch(c, "adi_coef_vz", [&] { hd_launch_adi_coef(c->d, 0, sidx, eidx, s); });
+            launch(c, "adi_solve_vz", [&] { hd_launch_adi_solve(c->d, 0, sidx, eidx, s); });
+        } else {
+            launch(c, "adi_line_vz", [&] { hd_launch_adi_line(c->d, 0, sidx, eidx, s); });
+        }
         launch(c, "smooth_v", [&] { hd_launch_normalize_post(c->d, 0, 0, 0, c->cand_cnt, c->cand, c->dryflag, s); });
         swap_v(c);
         launch(c, "explicit_u", [&] { hd_launch_explicit(c->d, 1, 2, M, s); });
@@ -453,9 +468,14 @@ static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sedim
     launch(c, "fs_edges", [&] { hd_launch_fs_edges(c->d, s); });
     if (sediment) {
         launch(c, "vth_hesok", [&] { hd_launch_vth_hesok(c->d, s); });
-        launch(c, which == 0 ? "fs_coef_j" : "fs_coef_i",
-               [&] { hd_launch_fs_coef(c->d, which, c->tdev, o.bed_change_start, o.ketdinh, s); });
-        launch(c, which == 0 ? "fs_solve_j" : "fs_solve_i", [&] { hd_launch_fs_solve(c->d, which, s); });
+        if (c->solver_v1) {
+            launch(c, which == 0 ? "fs_coef_j" : "fs_coef_i",
+                   [&] { hd_launch_fs_coef(c->d, which, c->tdev, o.bed_change_start, o.ketdinh, s); });
+            launch(c, which == 0 ? "fs_solve_j" : "fs_solve_i", [&] { hd_launch_fs_solve(c->d, which, s); });
+        } else {
+            launch(c, which == 0 ? "fs_line_j" : "fs_line_i",
+                   [&] { hd_launch_fs_line(c->d, which, c->tdev, o.bed_change_start, o.ketdinh, s); });
+        }
         launch(c, "update_fs", [&] { hd_launch_update_fs(c->d, s); });
         if (which == 1) { sidx = 3; eidx = N - 1; }
     }
@@ -612,9 +632,11 @@ int hd_wet_count(hd_ctx* c, long long* out)
 {
     if (!c || !out) return HD_EINVAL;
     unsigned long long z = 0;
-    HIPCHK(c, hipMemcpyAsync(c->d.wet_count, &z, 8, hipMemcpyHostToDevice, c->stream));
-    hd_launch_wet_count(c->d, c->stream);
-    HIPCHK(c, hipMemcpyAsync(out, c->d.wet_count, 8, hipMemcpyDeviceToHost, c->stream));
+    HdDev one = c->d;
+    one.wet_count = c->d.wet_count + 1;   /* leave the run accumulator alone */
+    HIPCHK(c, hipMemcpyAsync(one.wet_count, &z, 8, hipMemcpyHostToDevice, c->stream));
+    hd_launch_wet_count(one, c->stream);
+    HIPCHK(c, hipMemcpyAsync(out, one.wet_count, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return HD_OK;
 }

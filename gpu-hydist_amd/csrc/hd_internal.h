@@ -53,6 +53,7 @@ void hd_launch_preprocess(const HdDev& d, hipStream_t s);
 void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s);
 void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
 void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
 void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
                               unsigned char* dryflag, hipStream_t s);
 void hd_launch_explicit(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
@@ -63,6 +64,7 @@ void hd_launch_uvz_dry(const HdDev& d, int kd, int kn, const unsigned char* dryf
 void hd_launch_vth_hesok(const HdDev& d, hipStream_t s);
 void hd_launch_fs_coef(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_fs_solve(const HdDev& d, int which, hipStream_t s);
+void hd_launch_fs_line(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_update_fs(const HdDev& d, hipStream_t s);
 void hd_launch_calc_qb(const HdDev& d, int qb_mode, hipStream_t s);
 void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStream_t s);

@@ -621,6 +621,207 @@ __global__ void __launch_bounds__(64) k_adi_solve(HdDev d, int sidx, int eidx)
     }
 }
 
+/* The two tridiagonal rows (e = 0, 1) of position k of run [first, last],
+ * in the reference's interleaving (see k_adi_coef), with the start / end
+ * conditions of _calculate_matrix_coeff applied (end row wins). */
+struct Row4 {
+    double A, B, C, D;
+};
+
+__device__ __forceinline__ Row4 cell_row(const Pre& a, const Pre& b, double f4, int k, int e, int first, int last,
+                                         bool br1, bool br2, int skind, int ekind, int sn, double ub1, double ub2,
+                                         double TZ_l, double TZ_r)
+{
+    int rr = 2 * (k - first) + e;
+    Row4 w;
+    bool cont = (skind != S_OPEN) ? (e == 0) : (e == 1);
+    if (cont) {
+        const Pre& c = (skind != S_OPEN) ? a : b;
+        w.A = c.a2; w.B = 1; w.C = c.c2; w.D = c.d2;
+    } else {
+        Mom m = abcd(a, b, f4, k, first, last, br1, br2);
+        w.A = m.a1; w.B = m.b1; w.C = m.c1; w.D = m.d1;
+    }
+    if (rr == 0) {
+        if (skind == S_Q) w.D = a.d2 - a.a2 * ub1;
+        else if (skind == S_OPEN) w.D = w.D - w.A * TZ_l;
+    }
+    if (rr == sn) {
+        if (ekind == E_Q) {
+            const Pre& c = (k == last) ? a : b;
+            w.A = c.a2; w.B = 1; w.D = c.d2 - c.c2 * ub2;
+        } else if (ekind == E_OPEN) {
+            Mom m = abcd(a, b, f4, k, first, last, br1, br2);
+            w.A = m.a1; w.B = m.b1; w.D = m.d1 - m.c1 * TZ_r;
+        } else {
+            const Pre& c = (k == last) ? a : b;
+            w.A = c.a2; w.B = 1; w.D = c.d2;
+        }
+    }
+    return w;
+}
+
+/* Fused UZ / VZ line solve, one wave per line (the reference's
+ * _calculate_preindex + _calculate_abcd + _calculate_matrix_coeff +
+ * tridiagSolver + _extract_solution, UVZSolver_multithread.cu:203-723).
+ * For each wet run, chunks of 63 cells are assembled in parallel (lane k
+ * computes the preindex of its cell into LDS, then its two rows from its own
+ * and its neighbour's values), lane 0 carries the Thomas forward elimination
+ * through the chunk's rows in LDS, then back-substitutes, and all lanes
+ * scatter the solution into t_z / t_u (t_v).  Same operations in the same
+ * order as the sequential reference Thomas, so the results are bit-identical;
+ * only the schedule changes.  Elimination factors of rows beyond kCap spill
+ * to the global Ap/Bp line buffers. */
+template <bool isU>
+__global__ void __launch_bounds__(64) k_adi_line(HdDev d, int sidx, int eidx)
+{
+    constexpr int CH = 63;
+    constexpr int kCap = 2048;
+    __shared__ double sP[7][64];
+    __shared__ double sA[2 * CH], sB[2 * CH], sC[2 * CH], sD[2 * CH];
+    __shared__ double sAp[kCap], sBp[kCap];
+    __shared__ double s_end[6];
+    int line = sidx + blockIdx.x;
+    if (line > eidx) return;
+    int lane = threadIdx.x;
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    int nseg = isU ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    const int* dau = (isU ? d.dauj : d.daui) + line * SL;
+    const int* cuoi = (isU ? d.cuoij : d.cuoii) + line * SL;
+    bool dkQ1 = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0;
+    bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
+    int dkfr = isU ? N : M;
+    double ub1 = isU ? d.ubt[line] : d.vbd[line];
+    double ub2 = isU ? d.ubp[line] : d.vbt[line];
+    const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
+    double* tz = d.t_z;
+    double* tq = isU ? d.t_u : d.t_v;
+    double* gAp = d.Ap + (size_t)line * tw;
+    double* gBp = d.Bp + (size_t)line * tw;
+    for (int s = 0; s < nseg; ++s) {
+        int first = dau[s], last = cuoi[s];
+        bool br1, br2;
+        wall_flags(d, isU, line, first, last, br1, br2);
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
+        double TZ_l = isU ? tz[(size_t)first * W + line] : tz[(size_t)line * W + first];
+        double TZ_r = isU ? tz[(size_t)last * W + line] : tz[(size_t)line * W + last];
+        double* oAp = gAp + 2 * first;
+        double* oBp = gBp + 2 * first;
+        double xs = 0;
+        if (sn >= 0) {
+            double ap = 0, bp = 0;
+            for (int c0 = first; c0 <= last; c0 += CH) {
+                int k = c0 + lane;
+                Pre a;
+                if (k <= last) {
+                    a = isU ? pre_uz(d, k, line) : pre_vz(d, line, k);
+                    sP[0][lane] = a.f1; sP[1][lane] = a.f2; sP[2][lane] = a.f3; sP[3][lane] = a.f5;
+                    sP[4][lane] = a.a2; sP[5][lane] = a.c2; sP[6][lane] = a.d2;
+                    if (k == first) { s_end[0] = a.a2; s_end[1] = a.c2; s_end[2] = a.d2; }
+                    if (k == last) { s_end[3] = a.a2; s_end[4] = a.c2; s_end[5] = a.d2; }
+                }
+                __syncthreads();
+                if (lane < CH && k <= last) {
+                    Pre b = a;
+                    if (k + 1 <= last) {
+                        b.f1 = sP[0][lane + 1]; b.f2 = sP[1][lane + 1]; b.f3 = sP[2][lane + 1]; b.f5 = sP[3][lane + 1];
+                        b.a2 = sP[4][lane + 1]; b.c2 = sP[5][lane + 1]; b.d2 = sP[6][lane + 1];
+                    }
+                    for (int e = 0; e < 2; ++e) {
+                        int rr = 2 * (k - first) + e;
+                        if (rr > sn) break;
+                        Row4 w = cell_row(a, b, f4, k, e, first, last, br1, br2, skind, ekind, sn, ub1, ub2, TZ_l, TZ_r);
+                        sA[2 * lane + e] = w.A; sB[2 * lane + e] = w.B; sC[2 * lane + e] = w.C; sD[2 * lane + e] = w.D;
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    int rb = 2 * (c0 - first);
+                    int re = rb + 2 * CH - 1;
+                    if (re > sn) re = sn;
+                    for (int r = rb; r <= re; ++r) {
+                        double A = sA[r - rb], B = sB[r - rb], C = sC[r - rb], D = sD[r - rb];
+                        if (r == 0 && sn == 0) {
+                            xs = D / B;
+                        } else if (r == 0) {
+                            ap = -C / B;
+                            bp = D / B;
+                        } else if (r < sn) {
+                            double e = A * ap + B;
+                            double nap = -C / e;
+                            double nbp = (D - (A * bp)) / e;
+                            ap = nap;
+                            bp = nbp;
+                        } else {
+                            xs = (D - (A * bp)) / (B + (A * ap));
+                        }
+                        if (r < sn) {
+                            if (r < kCap) { sAp[r] = ap; sBp[r] = bp; }
+                            else { oAp[r] = ap; oBp[r] = bp; }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            /* back substitution; x[r] replaces Bp[r] */
+            if (lane == 0) {
+                double x = xs;
+                if (sn < kCap) sBp[sn] = x; else oBp[sn] = x;
+                for (int r = sn - 1; r >= 0; --r) {
+                    double apr = r < kCap ? sAp[r] : oAp[r];
+                    double bpr = r < kCap ? sBp[r] : oBp[r];
+                    x = bpr + (apr * x);
+                    if (r < kCap) sBp[r] = x; else oBp[r] = x;
+                }
+            }
+            __syncthreads();
+            /* scatter the unknowns of positions first .. last-1 */
+            for (int r = lane; r <= sn; r += 64) {
+                double x = r < kCap ? sBp[r] : oBp[r];
+                int k;
+                bool isz;
+                if (skind != S_OPEN) { k = first + (r >> 1); isz = (r & 1) == 0; }
+                else { isz = (r & 1) == 1; k = first + ((r + 1) >> 1); }
+                if (k < last) {
+                    size_t p = isU ? (size_t)k * W + line : (size_t)line * W + k;
+                    if (isz) tz[p] = x; else tq[p] = x;
+                }
+            }
+        } else if (lane == 0) {
+            /* D21: no system; the cell's preindex is still needed at the end */
+            Pre a = isU ? pre_uz(d, last, line) : pre_vz(d, line, last);
+            s_end[3] = a.a2; s_end[4] = a.c2; s_end[5] = a.d2;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            auto at = [&](int k) -> size_t { return isU ? (size_t)k * W + line : (size_t)line * W + k; };
+            auto X = [&](int r) -> double { return r < kCap ? sBp[r] : oBp[r]; };
+            bool exQ = dkQ2, exQs = dkQ1;
+            if (first < last) {
+                if (br1) tq[at(first - 1)] = 0;
+                else if (exQs && first == 2) tq[at(first - 1)] = ub1;
+                else if (sn >= 0) tq[at(first - 1)] = (s_end[2] - tz[at(first)] - s_end[1] * X(0)) / s_end[0];
+            }
+            if (br2) {
+                tq[at(last)] = 0;
+                if (sn >= 0) tz[at(last)] = X(sn);
+            } else if (exQ && last == dkfr) {
+                tq[at(last)] = ub2;
+                if (sn >= 0) tz[at(last)] = X(sn);
+            } else {
+                double prev;
+                if (first < last && sn >= 0) prev = X(skind != S_OPEN ? 2 * (last - 1 - first) + 1 : 2 * (last - 1 - first));
+                else prev = tq[at(last - 1)];
+                tq[at(last)] = (s_end[5] - s_end[3] * prev - tz[at(last)]) / s_end[4];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 /* ================================================================ smoothing */
 
 /* Normalize + update_buffer, support_funcs.cu:682-729, writing the partner
@@ -1151,6 +1352,14 @@ void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t
     if (eidx < sidx) return;
     if (isU) hipLaunchKernelGGL(k_adi_coef<true>, grd, blk, 0, s, d, sidx, eidx);
     else hipLaunchKernelGGL(k_adi_coef<false>, grd, blk, 0, s, d, sidx, eidx);
+}
+
+void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+{
+    int nl = eidx - sidx + 1;
+    if (nl <= 0) return;
+    if (isU) hipLaunchKernelGGL(k_adi_line<true>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_adi_line<false>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
 }
 
 void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)

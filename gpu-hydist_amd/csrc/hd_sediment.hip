@@ -83,6 +83,80 @@ __device__ __forceinline__ void wall_flags2(const HdDev& d, bool isJ, int line, 
     }
 }
 
+struct FsRow {
+    double A, B, C, D;
+};
+
+/* One concentration row at cell (i, j) of run [first, last]:
+ * _FSj_calculate__mactrix_coeff (Sediment_Transport.cu:205-302, column lines,
+ * gamav before Zf and && at the last row, as written) or
+ * _FSi_calculate__mactrix_coeff (:62-165, row lines). */
+template <bool isJ>
+__device__ FsRow fs_row(const HdDev& d, double t, double s_start, int ketdinh, int i, int j, int first, int last,
+                        bool br1, bool br2)
+{
+    const hd_params& P = d.P;
+    int W = d.W;
+    const double *FS = d.FS, *Hm = d.H_moi, *t_u = d.t_u, *t_v = d.t_v, *VTH = d.VTH, *Htdu = d.Htdu, *Htdv = d.Htdv,
+                 *Fw = d.Fw, *Kx = d.Kx, *Ky = d.Ky;
+        int pos = i * W + j;
+        double c = 18 * hd_log(12 * Hm[pos] / P.Ks);
+        double Uf = sqrt(P.g) * fabs(VTH[pos]) / c;
+        double wsm = P.wss * hd_pow((1 - FS[pos]), 4);
+        double Zf = 0, S = 0, gamav;
+        if (isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
+        if (FS[pos] > 0.00001) Zf = wsm / (0.4 * (Uf + 2 * wsm));
+        if (ketdinh) S = source_chs(P, wsm, Zf, FS[pos], Fw[pos], VTH[pos], Hm[pos]);
+        else S = source_nchs(P, wsm, Zf, Uf, Hm[pos], FS[pos]);
+        if (t < s_start) S = 0;
+        if (!isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
+        double A, B, C, D, p = 0, q = 0;
+        if (isJ) {
+            A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos - W] * Kx[pos - W] / (Hm[pos] * (P.dXbp));
+            C = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos] * Kx[pos] / (Hm[pos] * (P.dXbp));
+            B = P.HaiChiadT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (Hm[pos] * (P.dXbp));
+            if ((Hm[pos - 1] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) p = (FS[pos + 1] - FS[pos]);
+            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 1] > P.H_TINH)) q = (FS[pos] - FS[pos - 1]);
+            p = (1 / (Hm[pos] * P.dYbp)) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) {
+                q = (FS[pos + 1] - FS[pos - 1]) / P.dY2;
+                q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
+            } else q = 0;
+            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
+            if (i == first + 1) {
+                if ((br1) || (t_u[first * W + j] == 0)) B += A;
+                else D -= A * FS[first * W + j];
+            }
+            if (i == last - 1) {
+                if ((br2) && (t_u[last * W + j] == 0)) B += C;
+                else D -= C * FS[last * W + j];
+            }
+        } else {
+            A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos - 1] * Ky[pos - 1] / (Hm[pos] * P.dYbp);
+            C = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos] * Ky[pos] / (Hm[pos] * P.dYbp);
+            B = P.HaiChiadT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (Hm[pos] * P.dYbp);
+            if ((Hm[pos + 2 * W] > P.H_TINH) && (Hm[pos - W] > P.H_TINH)) p = (FS[pos + W] - FS[pos]);
+            if (Hm[pos - 2 * W] > P.H_TINH && Hm[pos + W] > P.H_TINH) q = (FS[pos] - FS[pos - W]);
+            p = (1 / (Hm[pos] * P.dXbp)) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+            if (Hm[pos + 2 * W] > P.H_TINH && Hm[pos - 2 * W] > P.H_TINH) {
+                q = (FS[pos + W] - FS[pos - W]) / P.dX2;
+                q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
+            } else q = 0;
+            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
+            if (j == first + 1) {
+                if ((br1) || (t_v[i * W + first] == 0)) B = B + A;
+                else D = D - A * FS[i * W + first];
+            }
+            if (j == last - 1) {
+                if ((br2) || (t_v[i * W + last] == 0)) B = B + C;
+                else D = D - C * FS[i * W + last];
+            }
+        }
+    FsRow w;
+    w.A = A; w.B = B; w.C = C; w.D = D;
+    return w;
+}
+
 /* Concentration rows: _FSj_calculate__mactrix_coeff (Sediment_Transport.cu:205-302,
  * column lines, gamav before Zf and && at the last row, as written) or
  * _FSi_calculate__mactrix_coeff (:62-165, row lines).  Row of cell k sits at
@@ -141,59 +215,8 @@ __global__ void __launch_bounds__(256) k_fs_coef(HdDev d, const double* tdev, do
         if (!found || last < first + 2 || k < first + 1 || k > last - 1) continue;
         bool br1, br2;
         wall_flags2(d, isJ, line, first, last, br1, br2);
-        int pos = i * W + j;
-        double c = 18 * hd_log(12 * Hm[pos] / P.Ks);
-        double Uf = sqrt(P.g) * fabs(VTH[pos]) / c;
-        double wsm = P.wss * hd_pow((1 - FS[pos]), 4);
-        double Zf = 0, S = 0, gamav;
-        if (isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
-        if (FS[pos] > 0.00001) Zf = wsm / (0.4 * (Uf + 2 * wsm));
-        if (ketdinh) S = source_chs(P, wsm, Zf, FS[pos], Fw[pos], VTH[pos], Hm[pos]);
-        else S = source_nchs(P, wsm, Zf, Uf, Hm[pos], FS[pos]);
-        if (t < s_start) S = 0;
-        if (!isJ) gamav = 0.98 - 0.198 * Zf + 0.032 * Zf * Zf;
-        double A, B, C, D, p = 0, q = 0;
-        if (isJ) {
-            A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos - W] * Kx[pos - W] / (Hm[pos] * (P.dXbp));
-            C = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / P.dX2 - Htdu[pos] * Kx[pos] / (Hm[pos] * (P.dXbp));
-            B = P.HaiChiadT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (Hm[pos] * (P.dXbp));
-            if ((Hm[pos - 1] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) p = (FS[pos + 1] - FS[pos]);
-            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 1] > P.H_TINH)) q = (FS[pos] - FS[pos - 1]);
-            p = (1 / (Hm[pos] * P.dYbp)) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
-            if ((Hm[pos - 2] > P.H_TINH) && (Hm[pos + 2] > P.H_TINH)) {
-                q = (FS[pos + 1] - FS[pos - 1]) / P.dY2;
-                q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
-            } else q = 0;
-            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
-            if (i == first + 1) {
-                if ((br1) || (t_u[first * W + j] == 0)) B += A;
-                else D -= A * FS[first * W + j];
-            }
-            if (i == last - 1) {
-                if ((br2) && (t_u[last * W + j] == 0)) B += C;
-                else D -= C * FS[last * W + j];
-            }
-        } else {
-            A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos - 1] * Ky[pos - 1] / (Hm[pos] * P.dYbp);
-            C = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / P.dY2 - Htdv[pos] * Ky[pos] / (Hm[pos] * P.dYbp);
-            B = P.HaiChiadT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (Hm[pos] * P.dYbp);
-            if ((Hm[pos + 2 * W] > P.H_TINH) && (Hm[pos - W] > P.H_TINH)) p = (FS[pos + W] - FS[pos]);
-            if (Hm[pos - 2 * W] > P.H_TINH && Hm[pos + W] > P.H_TINH) q = (FS[pos] - FS[pos - W]);
-            p = (1 / (Hm[pos] * P.dXbp)) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
-            if (Hm[pos + 2 * W] > P.H_TINH && Hm[pos - 2 * W] > P.H_TINH) {
-                q = (FS[pos + W] - FS[pos - W]) / P.dX2;
-                q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
-            } else q = 0;
-            D = FS[pos] / (P.dT * 0.5) - q + p + (S / Hm[pos]);
-            if (j == first + 1) {
-                if ((br1) || (t_v[i * W + first] == 0)) B = B + A;
-                else D = D - A * FS[i * W + first];
-            }
-            if (j == last - 1) {
-                if ((br2) || (t_v[i * W + last] == 0)) B = B + C;
-                else D = D - C * FS[i * W + last];
-            }
-        }
+        FsRow w = fs_row<isJ>(d, t, s_start, ketdinh, i, j, first, last, br1, br2);
+        double A = w.A, B = w.B, C = w.C, D = w.D;
         size_t o = (size_t)line * tw + k;
         d.AA[o] = A; d.BB[o] = B; d.CC[o] = C; d.DD[o] = D;
     }
@@ -255,6 +278,100 @@ __global__ void __launch_bounds__(64) k_fs_solve(HdDev d)
         else d.tFS[at(first)] = d.FS[at(first)];
         if ((br2) || (tq[at(last)] == 0)) d.tFS[at(last)] = vlast;
         else d.tFS[at(last)] = d.FS[at(last)];
+    }
+}
+
+/* Fused concentration sweep, one wave per line: rows of 64 interior cells
+ * at a time in parallel into LDS, lane 0 runs the Thomas recurrence
+ * (UVZSolver_multithread.cu:203-228, jump 1) over them, then back
+ * substitution and the extraction into tFS with the end copies
+ * (Sediment_Transport.cu:305-338 / 168-202), lanes in parallel. */
+template <bool isJ>
+__global__ void __launch_bounds__(64) k_fs_line(HdDev d, const double* tdev, double s_start, int ketdinh)
+{
+    constexpr int CH = 64;
+    constexpr int kCap = 2048;
+    __shared__ double sA[CH], sB[CH], sC[CH], sD[CH];
+    __shared__ double sAp[kCap], sBp[kCap];
+    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int line = 3 + blockIdx.x;
+    if (line > (isJ ? M - 1 : N - 1)) return;
+    int lane = threadIdx.x;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const double t = tdev[0];
+    int nseg = isJ ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    const int* dau = (isJ ? d.dauj : d.daui) + line * SL;
+    const int* cuoi = (isJ ? d.cuoij : d.cuoii) + line * SL;
+    double* gAp = d.Ap + (size_t)line * tw;
+    double* gBp = d.Bp + (size_t)line * tw;
+    auto at = [&](int k) -> size_t { return isJ ? (size_t)k * W + line : (size_t)line * W + k; };
+    for (int s = 0; s < nseg; ++s) {
+        int first = dau[s], last = cuoi[s];
+        if (last < first + 2) continue;
+        bool br1, br2;
+        wall_flags2(d, isJ, line, first, last, br1, br2);
+        int sn = last - first - 2;
+        int k0 = first + 1;
+        double* oAp = gAp + k0;
+        double* oBp = gBp + k0;
+        double ap = 0, bp = 0, xs = 0;
+        for (int c0 = 0; c0 <= sn; c0 += CH) {
+            int r = c0 + lane;
+            if (r <= sn) {
+                int k = k0 + r;
+                int i = isJ ? k : line, j = isJ ? line : k;
+                FsRow w = fs_row<isJ>(d, t, s_start, ketdinh, i, j, first, last, br1, br2);
+                sA[lane] = w.A; sB[lane] = w.B; sC[lane] = w.C; sD[lane] = w.D;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int re = c0 + CH - 1;
+                if (re > sn) re = sn;
+                for (int q = c0; q <= re; ++q) {
+                    double A = sA[q - c0], B = sB[q - c0], C = sC[q - c0], D = sD[q - c0];
+                    if (q == 0 && sn == 0) xs = D / B;
+                    else if (q == 0) { ap = -C / B; bp = D / B; }
+                    else if (q < sn) {
+                        double e = A * ap + B;
+                        double nap = -C / e;
+                        double nbp = (D - (A * bp)) / e;
+                        ap = nap; bp = nbp;
+                    } else xs = (D - (A * bp)) / (B + (A * ap));
+                    if (q < sn) {
+                        if (q < kCap) { sAp[q] = ap; sBp[q] = bp; }
+                        else { oAp[q] = ap; oBp[q] = bp; }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            double x = xs;
+            if (sn < kCap) sBp[sn] = x; else oBp[sn] = x;
+            for (int q = sn - 1; q >= 0; --q) {
+                double apq = q < kCap ? sAp[q] : oAp[q];
+                double bpq = q < kCap ? sBp[q] : oBp[q];
+                x = bpq + (apq * x);
+                if (q < kCap) sBp[q] = x; else oBp[q] = x;
+            }
+        }
+        __syncthreads();
+        for (int q = lane; q <= sn; q += 64) {
+            double v = q < kCap ? sBp[q] : oBp[q];
+            if (v < 0) v = d.P.NDnen;
+            d.tFS[at(k0 + q)] = v;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double vfirst = d.tFS[at(k0)], vlast = d.tFS[at(last - 1)];
+            const double* tqq = isJ ? d.t_u : d.t_v;
+            if ((br1) || (tqq[at(first)] == 0)) d.tFS[at(first)] = vfirst;
+            else d.tFS[at(first)] = d.FS[at(first)];
+            if ((br2) || (tqq[at(last)] == 0)) d.tFS[at(last)] = vlast;
+            else d.tFS[at(last)] = d.FS[at(last)];
+        }
+        __syncthreads();
     }
 }
 
@@ -411,4 +528,12 @@ void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStrea
 {
     int nr = eidx - sidx;
     if (nr > 0) hipLaunchKernelGGL(k_bedload, dim3((d.M - 1 + 255) / 256, nr), dim3(256), 0, s, d, ketdinh, sidx, eidx);
+}
+
+void hd_launch_fs_line(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s)
+{
+    int nl = (which == 0 ? d.M : d.N) - 3;
+    if (nl <= 0) return;
+    if (which == 0) hipLaunchKernelGGL(k_fs_line<true>, dim3(nl), dim3(64), 0, s, d, tdev, s_start, ketdinh);
+    else hipLaunchKernelGGL(k_fs_line<false>, dim3(nl), dim3(64), 0, s, d, tdev, s_start, ketdinh);
 }
