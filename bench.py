@@ -31,6 +31,9 @@ B_STEP = 256               # algorithmic bytes per wet-cell-step (hydrodynamics)
 KERNEL_BYTES = {
     # fused assembly + Thomas + extraction of one half-step: u, v, z, Htdu,
     # Htdv, H_moi, Kx1|Ky1, VISCOIDX, Tsxw|Tsyw read, t_z and t_u|t_v written
+    "adi_run_uz": 9 * 8 + 2 * 8,
+    "adi_run_vz": 9 * 8 + 2 * 8,
+    # the same work, one wave per line (HYDIST_SOLVER=line)
     "adi_line_uz": 9 * 8 + 2 * 8,
     "adi_line_vz": 9 * 8 + 2 * 8,
     # two-kernel form (HYDIST_SOLVER=rows): rows of A,B,C,D round-trip HBM

@@ -84,7 +84,12 @@ struct hd_ctx {
     bool initialised = false;
     std::string err;
     std::map<GraphKey, hipGraphExec_t> graphs;
-    bool solver_v1 = false;     /* two-kernel rows + lane-per-line solve (HYDIST_SOLVER=rows) */
+    /* hydrodynamic solver form (HYDIST_SOLVER): 0 = one wave per wet run
+     * (default), 1 = one wave per line ("line"), 2 = rows kernel + one lane
+     * per line ("rows").  All three give identical bits. */
+    int solver = 0;
+    bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
+    int run_blocks_u = 0, run_blocks_v = 0;
     /* per-kernel HIP-event timing (hd_profile / hd_kernel_times) */
     bool profile = false;
     struct Rec { const char* name; hipEvent_t a, b; };
@@ -195,6 +200,7 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     {
         const char* sv = getenv("HYDIST_SOLVER");
         c->solver_v1 = sv && std::string(sv) == "rows";
+        c->solver = c->solver_v1 ? 2 : (sv && std::string(sv) == "line") ? 1 : 0;
     }
     int rc;
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, HD_EHIP, "hipSetDevice failed"); }
@@ -248,6 +254,13 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
         if ((rc = dalloc(c, (void**)r, tw * nlines * 8))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.seg_end, nlines * SL * 6 * 8))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.seg_sn, nlines * SL * 4))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.runs_u, (size_t)(M + 3) * SL * sizeof(int4)))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.runs_v, (size_t)(N + 3) * SL * sizeof(int4)))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.run_cnt, 4 * 4))) { hd_destroy(c); return rc; }
+    /* one block per run while the table holds up to twice as many runs as
+       lines; blocks stride over the rest */
+    c->run_blocks_u = 2 * (M + 3) < (M + 3) * SL ? 2 * (M + 3) : (M + 3) * SL;
+    c->run_blocks_v = 2 * (N + 3) < (N + 3) * SL ? 2 * (N + 3) : (N + 3) * SL;
     if ((rc = dalloc(c, (void**)&d.flags, 16 * 4))) { hd_destroy(c); return rc; }
     /* [0]: per-step accumulator of hd_run_counted, [1]: one-shot count of hd_wet_count */
     if ((rc = dalloc(c, (void**)&d.wet_count, 16))) { hd_destroy(c); return rc; }
@@ -433,11 +446,13 @@ static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sedim
     if (which == 0) {
         sidx = 2; eidx = M;
         if (channel && o.kenhhepd == 1) { sidx = 3; eidx = M - 1; }
-        if (c->solver_v1) {
+        if (c->solver == 2) {
             launch(c, "adi_coef_uz", [&] { hd_launch_adi_coef(c->d, 1, sidx, eidx, s); });
             launch(c, "adi_solve_uz", [&] { hd_launch_adi_solve(c->d, 1, sidx, eidx, s); });
-        } else {
+        } else if (c->solver == 1) {
             launch(c, "adi_line_uz", [&] { hd_launch_adi_line(c->d, 1, sidx, eidx, s); });
+        } else {
+            launch(c, "adi_run_uz", [&] { hd_launch_adi_run(c->d, 1, sidx, eidx, c->run_blocks_u, s); });
         }
         launch(c, "smooth_u", [&] { hd_launch_normalize_post(c->d, 1, 0, 1, c->cand_cnt, c->cand, c->dryflag, s); });
         swap_u(c);
@@ -449,11 +464,13 @@ static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sedim
     } else {
         sidx = 2; eidx = N;
         if (channel && o.kenhhepng == 1) { sidx = 3; eidx = N; }
-        if (c->solver_v1) {
+        if (c->solver == 2) {
             launch(c, "adi_coef_vz", [&] { hd_launch_adi_coef(c->d, 0, sidx, eidx, s); });
             launch(c, "adi_solve_vz", [&] { hd_launch_adi_solve(c->d, 0, sidx, eidx, s); });
-        } else {
+        } else if (c->solver == 1) {
             launch(c, "adi_line_vz", [&] { hd_launch_adi_line(c->d, 0, sidx, eidx, s); });
+        } else {
+            launch(c, "adi_run_vz", [&] { hd_launch_adi_run(c->d, 0, sidx, eidx, c->run_blocks_v, s); });
         }
         launch(c, "smooth_v", [&] { hd_launch_normalize_post(c->d, 0, 0, 0, c->cand_cnt, c->cand, c->dryflag, s); });
         swap_v(c);

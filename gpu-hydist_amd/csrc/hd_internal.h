@@ -39,6 +39,8 @@ struct HdDev {
     double *AA, *BB, *CC, *DD, *Ap, *Bp;
     double* seg_end;                /* per line*SL+seg: a2f,c2f,d2f,a2l,c2l,d2l */
     int* seg_sn;                    /* per line*SL+seg: system size sn */
+    int4 *runs_u, *runs_v;          /* packed wet runs (line, first, last, 0) of the UZ / VZ sweeps */
+    int* run_cnt;                   /* [0]: entries of runs_u, [1]: entries of runs_v */
     int* flags;                     /* [0]: segment overflow, [1]: boundary range */
     long long* wet_count;
 };
@@ -53,6 +55,7 @@ void hd_launch_preprocess(const HdDev& d, hipStream_t s);
 void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s);
 void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
 void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
+void hd_launch_adi_run(const HdDev& d, int isU, int sidx, int eidx, int nblocks, hipStream_t s);
 void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
 void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
                               unsigned char* dryflag, hipStream_t s);

@@ -822,6 +822,197 @@ __global__ void __launch_bounds__(64) k_adi_line(HdDev d, int sidx, int eidx)
     }
 }
 
+/* Fused UZ / VZ solve, one wave per wet run taken from the packed run table
+ * that k_find_limits appends to (runs_u: column lines of the UZ half-step,
+ * runs_v: row lines of the VZ half-step).  Same operations in the same order
+ * as k_adi_line (and the reference's _calculate_preindex ... tridiagSolver ...
+ * _extract_solution, UVZSolver_multithread.cu:203-723); runs of one line are
+ * independent systems, so they now proceed on different waves and the
+ * critical path of the launch is the longest run, not the longest line.
+ * Chunks of 127 cells (254 rows) are assembled by the 64 lanes (two cells
+ * each) into LDS; lane 0 carries the forward elimination through the chunk,
+ * overwriting A/B with the factors ap/bp; the factors of all but the last
+ * chunk go to the run's slice of the Ap/Bp line buffers and come back for
+ * back substitution.  The end conditions take X(0), X(sn) and the last
+ * interior u unknown from the solution held in LDS, and the unchanged boundary
+ * depths TZ_l / TZ_r loaded at the start, so no lane reads a value another
+ * lane of this launch wrote. */
+template <bool isU>
+__global__ void __launch_bounds__(64) k_adi_run(HdDev d, int sidx, int eidx)
+{
+    constexpr int CH = 127;
+    constexpr int RW = 2 * CH;
+    __shared__ double sP[7][CH + 1];
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];
+    __shared__ double sE[6];
+    __shared__ double sX[3]; /* X(0), X(sn), X(last interior u row) */
+    const int lane = threadIdx.x;
+    const int N = d.N, M = d.M, W = d.W;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const int cap = (isU ? M + 3 : N + 3) * d.SL;
+    const int cnt = d.run_cnt[isU ? 0 : 1] < cap ? d.run_cnt[isU ? 0 : 1] : cap;
+    const int4* runs = isU ? d.runs_u : d.runs_v;
+    const bool dkQ1 = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0;
+    const bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
+    const int dkfr = isU ? N : M;
+    const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
+    double* tz = d.t_z;
+    double* tq = isU ? d.t_u : d.t_v;
+    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+        const int4 rn = runs[ri];
+        const int line = rn.x, first = rn.y, last = rn.z;
+        if (line < sidx || line > eidx) continue;
+        auto at = [&](int k) -> size_t { return isU ? (size_t)k * W + line : (size_t)line * W + k; };
+        bool br1, br2;
+        wall_flags(d, isU, line, first, last, br1, br2);
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
+        const double ub1 = isU ? d.ubt[line] : d.vbd[line];
+        const double ub2 = isU ? d.ubp[line] : d.vbt[line];
+        const double TZ_l = tz[at(first)];
+        const double TZ_r = tz[at(last)];
+        /* row of the u unknown of cell last-1 (read by the open-end rule) */
+        const int ru = (skind != S_OPEN) ? 2 * (last - 1 - first) + 1 : 2 * (last - 1 - first);
+        double* gAp = d.Ap + (size_t)line * tw + 2 * first;
+        double* gBp = d.Bp + (size_t)line * tw + 2 * first;
+        if (sn >= 0) {
+            const int nch = sn / RW + 1; /* chunks of RW rows; cells of chunk c: first + CH*c ... */
+            double ap = 0, bp = 0, xs = 0;
+            for (int c = 0; c < nch; ++c) {
+                const int c0 = first + CH * c;
+                const int rb = RW * c;
+                const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+                Pre pa[2];
+                for (int h = 0; h < 2; ++h) {
+                    int t = lane + 64 * h;
+                    int k = c0 + t;
+                    if (k <= last) {
+                        Pre a = isU ? pre_uz(d, k, line) : pre_vz(d, line, k);
+                        pa[h] = a;
+                        sP[0][t] = a.f1; sP[1][t] = a.f2; sP[2][t] = a.f3; sP[3][t] = a.f5;
+                        sP[4][t] = a.a2; sP[5][t] = a.c2; sP[6][t] = a.d2;
+                        if (k == first) { sE[0] = a.a2; sE[1] = a.c2; sE[2] = a.d2; }
+                        if (k == last) { sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2; }
+                    }
+                }
+                __syncthreads();
+                for (int h = 0; h < 2; ++h) {
+                    int t = lane + 64 * h;
+                    int k = c0 + t;
+                    if (t < CH && k <= last) {
+                        const Pre& a = pa[h];
+                        Pre b = a;
+                        if (k + 1 <= last) {
+                            b.f1 = sP[0][t + 1]; b.f2 = sP[1][t + 1]; b.f3 = sP[2][t + 1]; b.f5 = sP[3][t + 1];
+                            b.a2 = sP[4][t + 1]; b.c2 = sP[5][t + 1]; b.d2 = sP[6][t + 1];
+                        }
+                        for (int e = 0; e < 2; ++e) {
+                            int rr = 2 * (k - first) + e;
+                            if (rr > sn) break;
+                            Row4 w = cell_row(a, b, f4, k, e, first, last, br1, br2, skind, ekind, sn, ub1, ub2, TZ_l,
+                                              TZ_r);
+                            int ix = rr - rb;
+                            sA[ix] = w.A; sB[ix] = w.B; sC[ix] = w.C; sD[ix] = w.D;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    int i = 0;
+                    if (rb == 0) {
+                        if (sn == 0) {
+                            xs = sD[0] / sB[0];
+                        } else {
+                            ap = -sC[0] / sB[0];
+                            bp = sD[0] / sB[0];
+                            sA[0] = ap; sB[0] = bp;
+                        }
+                        i = 1;
+                    }
+                    const int imid = ((re < sn - 1) ? re : sn - 1) - rb;
+#pragma unroll 4
+                    for (; i <= imid; ++i) {
+                        double A = sA[i], B = sB[i], C = sC[i], D = sD[i];
+                        double e = A * ap + B;
+                        double nap = -C / e;
+                        double nbp = (D - (A * bp)) / e;
+                        ap = nap;
+                        bp = nbp;
+                        sA[i] = ap; sB[i] = bp;
+                    }
+                    if (re == sn && sn > 0) {
+                        double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
+                        xs = (D - (A * bp)) / (B + (A * ap));
+                    }
+                }
+                __syncthreads();
+                if (c + 1 < nch) { /* factors of this chunk (rows rb .. re, all < sn) to the line buffers */
+                    for (int i = lane; i <= re - rb; i += 64) { gAp[rb + i] = sA[i]; gBp[rb + i] = sB[i]; }
+                }
+            }
+            /* back substitution, last chunk first; x[r] replaces D[r] */
+            for (int c = nch - 1; c >= 0; --c) {
+                const int rb = RW * c;
+                const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+                if (c + 1 < nch) {
+                    __syncthreads();
+                    for (int i = lane; i <= re - rb; i += 64) { sA[i] = gAp[rb + i]; sB[i] = gBp[rb + i]; }
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    int i = re - rb;
+                    if (c == nch - 1) { sD[i] = xs; --i; }
+#pragma unroll 4
+                    for (; i >= 0; --i) {
+                        xs = sB[i] + (sA[i] * xs);
+                        sD[i] = xs;
+                    }
+                }
+                __syncthreads();
+                /* scatter the unknowns of positions first .. last-1 */
+                for (int i = lane; i <= re - rb; i += 64) {
+                    int r = rb + i;
+                    double x = sD[i];
+                    int k;
+                    bool isz;
+                    if (skind != S_OPEN) { k = first + (r >> 1); isz = (r & 1) == 0; }
+                    else { isz = (r & 1) == 1; k = first + ((r + 1) >> 1); }
+                    if (k < last) {
+                        if (isz) tz[at(k)] = x; else tq[at(k)] = x;
+                    }
+                    if (r == 0) sX[0] = x;
+                    if (r == sn) sX[1] = x;
+                    if (r == ru) sX[2] = x;
+                }
+            }
+        } else if (lane == 0) {
+            /* no system (one-cell run at an open start edge): the cell's
+             * preindex is still needed at the end */
+            Pre a = isU ? pre_uz(d, last, line) : pre_vz(d, line, last);
+            sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            if (first < last) {
+                if (br1) tq[at(first - 1)] = 0;
+                else if (dkQ1 && first == 2) tq[at(first - 1)] = ub1;
+                else if (sn >= 0) tq[at(first - 1)] = (sE[2] - TZ_l - sE[1] * sX[0]) / sE[0];
+            }
+            if (br2) {
+                tq[at(last)] = 0;
+                if (sn >= 0) tz[at(last)] = sX[1];
+            } else if (dkQ2 && last == dkfr) {
+                tq[at(last)] = ub2;
+                if (sn >= 0) tz[at(last)] = sX[1];
+            } else {
+                double prev = (first < last && sn >= 0) ? sX[2] : tq[at(last - 1)];
+                tq[at(last)] = (sE[5] - sE[3] * prev - TZ_r) / sE[4];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 /* ================================================================ smoothing */
 
 /* Normalize + update_buffer, support_funcs.cu:682-729, writing the partner
@@ -860,13 +1051,17 @@ __global__ void __launch_bounds__(256) k_normalize(HdDev d, int isU, int post, i
         dry = inrange && (hm <= d.P.H_TINH);
     }
     dryflag[p] = dry;
+    /* a position can open a neighbour only where t_z > z; t_z changes
+     * during the wetting pass only at cells it opens, and those are reached
+     * by the pass's own cascade */
+    bool rises = d.t_z[p] > d.z[p];
     if (horizontal) {
-        if (i >= 1 && i <= N && j >= 2 && j <= M && (d.khouot[p - 1] == 1 || d.khouot[p + 1] == 1)) {
+        if (i >= 1 && i <= N && j >= 2 && j <= M && rises && (d.khouot[p - 1] == 1 || d.khouot[p + 1] == 1)) {
             int slot = atomicAdd(&cand_cnt[i], 1);
             if (slot < kCand) cand[i * kCand + slot] = j;
         }
     } else {
-        if (j >= 1 && j <= M && i >= 2 && i <= N && (d.khouot[p - W] == 1 || d.khouot[p + W] == 1)) {
+        if (j >= 1 && j <= M && i >= 2 && i <= N && rises && (d.khouot[p - W] == 1 || d.khouot[p + W] == 1)) {
             int slot = atomicAdd(&cand_cnt[j], 1);
             if (slot < kCand) cand[j * kCand + slot] = i;
         }
@@ -1087,16 +1282,26 @@ __global__ void k_wetting(HdDev d, int horizontal, int* cand_cnt, int* cand, uns
         bool inrange = horizontal ? (q >= 2 && q <= M && line >= 1 && line <= N) : (q >= 2 && q <= N && line >= 1 && line <= M);
         dryflag[pq] = inrange && (Hm[pq] <= d.P.H_TINH);
     };
-    int last_done = -1;
-    int total = full ? (pmax - 1) : cnt;
-    for (int t = 0; t < total; ++t) {
-        int pos = full ? (2 + t) : list[t];
-        if (pos == last_done) continue;
-        last_done = pos;
+    /* visit positions in scan order: the recorded ones, and after a cell
+     * opens its right neighbour, that neighbour (the reference scan's cascade) */
+    int li = 0, pos = 1;
+    bool cascade = false;
+    while (true) {
+        if (cascade) {
+            ++pos;
+        } else if (full) {
+            ++pos;
+        } else {
+            while (li < cnt && list[li] <= pos) ++li;
+            if (li >= cnt) break;
+            pos = list[li];
+        }
+        if (pos < 2 || pos > pmax) break;
+        cascade = false;
         int pp = at(pos);
         if (tz[pp] > z[pp]) {
             if (kh[at(pos - 1)] == 1) open(pos - 1, pos);
-            if (kh[at(pos + 1)] == 1) open(pos + 1, pos);
+            if (kh[at(pos + 1)] == 1) { open(pos + 1, pos); cascade = true; }
         }
     }
 }
@@ -1114,6 +1319,7 @@ __global__ void __launch_bounds__(256) k_uvz(HdDev d, int kd, int kn, const unsi
     int N = d.N, M = d.M, W = d.W;
     if (j >= M + 3) return;
     int p = i * W + j;
+    if (p == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; } /* the run tables are rebuilt next */
     bool me = dryflag[p];
     bool dn = (i + 1 <= N + 2) ? dryflag[p + W] : false;
     bool rt = (j + 1 <= M + 2) ? dryflag[p + 1] : false;
@@ -1172,87 +1378,122 @@ __device__ void fs_d(const HdDev& d, int location, int offset, int sign, int i)
     }
 }
 
-/* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335).
- * A wet run within [2, lim] starting at a <= lim-2 is a segment; its end is
- * its last wet cell (the serial scan's rules, including dropping a run that
- * starts at lim-1).  Blocks: [0, nrowb) one wave per row (ballots along the
- * row), [nrowb, nrowb+ncolb) one wave per column (lanes along the column),
- * last block: the FS edge interpolations (rows N and 2, then column M, as the
- * two reference kernels order them).  With count_wet the row waves also add
- * the row's khouot == 0 cells to the wet-cell accumulator. */
-__global__ void __launch_bounds__(64) k_find_limits(HdDev d, int nrowb, int ncolb, int count_wet)
+__global__ void k_reset_runs(HdDev d)
 {
-    int N = d.N, M = d.M, W = d.W, SL = d.SL;
-    int lane = threadIdx.x;
+    if (threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+}
+
+/* Wet runs of one line (a row when ROW, else a column), positions 2..L, by
+ * one wave: the line is read four 64-cell chunks at a time (all loads of a
+ * batch in flight together), neighbours come from lane shuffles, starts and
+ * ends are compacted with ballots.  The rule is the serial scan's
+ * (support_funcs.cu:252-279): a run must start at <= L-2; it ends at its last
+ * wet cell, except that a run reaching L-1 or L only counts when L-2 is wet.
+ * Entries k < SL go to dau / cuoi (global) and sf / sl (LDS); returns the
+ * number of runs found. */
+template <bool ROW>
+__device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, int SL, int lane, int* dau,
+                                         int* cuoi, int* sf, int* sl)
+{
+    auto wetv = [&](int x) -> int {
+        return (x >= 2 && x <= L) ? (kh[ROW ? line * W + x : x * W + line] == 0) : 0;
+    };
+    const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
+    int ns = 0, ne = 0;
+    int carry = 0; /* wet(first position of the chunk - 1) */
+    for (int b0 = 2; b0 <= L; b0 += 4 * kWave) {
+        int w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = wetv(b0 + kWave * u + lane);
+        const int after = wetv(b0 + 4 * kWave); /* the cell after the batch */
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (b0 + kWave * u > L) break;
+            const int x = b0 + kWave * u + lane;
+            const int up = __shfl_up(w[u], 1, kWave);
+            const int dn = __shfl_down(w[u], 1, kWave);
+            const int nx0 = (u < 3) ? __shfl(w[u < 3 ? u + 1 : u], 0, kWave) : after;
+            const int last_of_chunk = __shfl(w[u], kWave - 1, kWave);
+            const int prevw = lane == 0 ? carry : up;
+            const int nextw = lane == kWave - 1 ? nx0 : dn;
+            carry = last_of_chunk;
+            const bool w0 = w[u] != 0;
+            const bool st = w0 && (x == 2 || !prevw) && x <= L - 2;
+            const bool en = w0 && !nextw && (x <= L - 2 || (x == L - 1 && wLm2) || (x == L && wLm1 && wLm2));
+            const unsigned long long sm = __ballot(st), em = __ballot(en);
+            if (st) {
+                int k = ns + __popcll(sm & lanemask_lt());
+                if (k < SL) { dau[k] = x; sf[k] = x; }
+            }
+            if (en) {
+                int k = ne + __popcll(em & lanemask_lt());
+                if (k < SL) { cuoi[k] = x; sl[k] = x; }
+            }
+            ns += __popcll(sm);
+            ne += __popcll(em);
+        }
+    }
+    return ns;
+}
+
+/* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335):
+ * the per-line tables daui / cuoii / moci (rows i = 2..N) and dauj / cuoij /
+ * mocj (columns j = 2..M), plus the packed run tables the solvers read
+ * (runs_v for rows, runs_u for columns; their counters were zeroed by the
+ * previous kernel on the stream).  Blocks of four waves, one line per wave;
+ * each block reserves its table slots with one atomic.  Blocks [0, nrowb):
+ * rows 4b .. 4b+3; with count_wet these waves also add the row's
+ * khouot == 0 cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
+ * [nrowb, nrowb + ncolb): columns 2 + 4b' ... */
+__global__ void __launch_bounds__(256) k_find_limits(HdDev d, int nrowb, int ncolb, int count_wet)
+{
+    __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
+    const int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int* kh = d.khouot;
-    int b = blockIdx.x;
-    if (b < nrowb) {
-        int i = b; /* rows 0..N+2 for counting, 2..N for segments */
-        if (count_wet) {
+    const bool rows = blockIdx.x < (unsigned)nrowb;
+    int n = 0, line;
+    if (rows) {
+        const int i = 4 * blockIdx.x + wv; /* rows 0..N+2 for counting, 2..N for runs */
+        line = i;
+        if (count_wet && i <= N + 2) {
             int c = 0;
             for (int j = lane; j < M + 3; j += kWave) c += (kh[i * W + j] == 0);
             for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
             if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
         }
-        if (i < 2 || i > N) return;
-        const int* row = kh + i * W;
-        auto wet = [&](int x) -> bool { return x >= 2 && x <= M && row[x] == 0; };
-        int ns = 0, ne = 0;
-        for (int c0 = 2; c0 <= M; c0 += kWave) {
-            int j = c0 + lane;
-            bool w0 = wet(j);
-            bool st = w0 && (j == 2 || !wet(j - 1)) && j <= M - 2;
-            bool en = w0 && !wet(j + 1) &&
-                      (j <= M - 2 || (j == M - 1 && wet(M - 2)) || (j == M && wet(M - 1) && wet(M - 2)));
-            unsigned long long sm = __ballot(st), em = __ballot(en);
-            if (st) {
-                int k = ns + __popcll(sm & lanemask_lt());
-                if (k < SL) d.daui[i * SL + k] = j;
+        if (i >= 2 && i <= N) {
+            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv]);
+            n = ns < SL ? ns : SL;
+            if (lane == 0) {
+                d.moci[i] = n;
+                if (ns > SL) d.flags[0] = 1;
             }
-            if (en) {
-                int k = ne + __popcll(em & lanemask_lt());
-                if (k < SL) d.cuoii[i * SL + k] = j;
+        }
+    } else {
+        const int j = 2 + 4 * (blockIdx.x - nrowb) + wv;
+        line = j;
+        if (j <= M) {
+            int ns = scan_runs<false>(kh, W, j, N, SL, lane, d.dauj + j * SL, d.cuoij + j * SL, s_f[wv], s_l[wv]);
+            n = ns < SL ? ns : SL;
+            if (lane == 0) {
+                d.mocj[j] = n;
+                if (ns > SL) d.flags[0] = 1;
             }
-            ns += __popcll(sm);
-            ne += __popcll(em);
         }
-        if (lane == 0) {
-            d.moci[i] = ns < SL ? ns : SL;
-            if (ns > SL) d.flags[0] = 1;
-        }
-        return;
     }
-    b -= nrowb;
-    if (b < ncolb) {
-        int j = 2 + b;
-        if (j > M) return;
-        auto wet = [&](int x) -> bool { return x >= 2 && x <= N && kh[x * W + j] == 0; };
-        int ns = 0, ne = 0;
-        for (int c0 = 2; c0 <= N; c0 += kWave) {
-            int i = c0 + lane;
-            bool w0 = wet(i);
-            bool st = w0 && (i == 2 || !wet(i - 1)) && i <= N - 2;
-            bool en = w0 && !wet(i + 1) &&
-                      (i <= N - 2 || (i == N - 1 && wet(N - 2)) || (i == N && wet(N - 1) && wet(N - 2)));
-            unsigned long long sm = __ballot(st), em = __ballot(en);
-            if (st) {
-                int k = ns + __popcll(sm & lanemask_lt());
-                if (k < SL) d.dauj[j * SL + k] = i;
-            }
-            if (en) {
-                int k = ne + __popcll(em & lanemask_lt());
-                if (k < SL) d.cuoij[j * SL + k] = i;
-            }
-            ns += __popcll(sm);
-            ne += __popcll(em);
-        }
-        if (lane == 0) {
-            d.mocj[j] = ns < SL ? ns : SL;
-            if (ns > SL) d.flags[0] = 1;
-        }
-        return;
+    if (lane == 0) s_n[wv] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
+        int base = tot ? atomicAdd(d.run_cnt + (rows ? 1 : 0), tot) : 0;
+        for (int w = 0; w < 4; ++w) { s_base[w] = base; base += s_n[w]; }
     }
-    /* FS edge interpolation block (runs with 256 threads requested by host) */
+    __syncthreads();
+    const int cap = rows ? (N + 3) * SL : (M + 3) * SL;
+    int4* table = rows ? d.runs_v : d.runs_u;
+    const int at = s_base[wv] + lane;
+    if (lane < n && at < cap) table[at] = make_int4(line, s_f[wv][lane], s_l[wv][lane], 0);
 }
 
 __global__ void __launch_bounds__(256) k_fs_edges(HdDev d)
@@ -1326,13 +1567,15 @@ void hd_launch_preprocess(const HdDev& d, hipStream_t s)
 
 void hd_launch_find_limits(const HdDev& d, hipStream_t s)
 {
+    hipLaunchKernelGGL(k_reset_runs, dim3(1), dim3(64), 0, s, d);
     hd_launch_segments(d, 0, s);
     hd_launch_fs_edges(d, s);
 }
 
 void hd_launch_segments(const HdDev& d, int count, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_find_limits, dim3((d.N + 3) + (d.M - 1)), dim3(64), 0, s, d, d.N + 3, d.M - 1, count);
+    int nrowb = (d.N + 3 + 3) / 4, ncolb = (d.M - 1 + 3) / 4;
+    hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb), dim3(256), 0, s, d, nrowb, ncolb, count);
 }
 
 void hd_launch_fs_edges(const HdDev& d, hipStream_t s)
@@ -1360,6 +1603,13 @@ void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t
     if (nl <= 0) return;
     if (isU) hipLaunchKernelGGL(k_adi_line<true>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
     else hipLaunchKernelGGL(k_adi_line<false>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
+}
+
+void hd_launch_adi_run(const HdDev& d, int isU, int sidx, int eidx, int nblocks, hipStream_t s)
+{
+    if (eidx < sidx) return;
+    if (isU) hipLaunchKernelGGL(k_adi_run<true>, dim3(nblocks), dim3(64), 0, s, d, sidx, eidx);
+    else hipLaunchKernelGGL(k_adi_run<false>, dim3(nblocks), dim3(64), 0, s, d, sidx, eidx);
 }
 
 void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)

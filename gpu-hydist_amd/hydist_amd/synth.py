@@ -301,3 +301,31 @@ def config(n: int, scale: float = 1.0, hours: Optional[float] = None, **kw) -> C
                           q_mean=kw.pop("q_mean", 3000.0 * scale), q_amp=kw.pop("q_amp", 1000.0 * scale),
                           sediment=(n == 3), name=name, **kw)
     raise ValueError(f"no generator for config {n}")
+
+
+def ensemble_cases(n: int = 64, seed: int = 4, hours: float = 1.0, side_min: int = 256, side_max: int = 1024,
+                   sediment: bool = True):
+    """The batched-ensemble workload: `n` independent meandering rivers with
+    grid sides N, M drawn uniformly from [side_min, side_max] and, per
+    scenario, a constant discharge Q ~ U[500, 5000] m3/s, Manning n ~
+    U[0.02, 0.04] and tide amplitude ~ U[0, 1] m; dx = 20 m, dT = 5 s,
+    suspended sediment from t = 0 (bed change after half an hour).  The mask
+    is the 980 x 1048 river's, its meander scaled to the grid; channel widths
+    stay 30-60 cells so every discharge stays subcritical."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n):
+        N = int(rng.integers(side_min, side_max + 1))
+        M = int(rng.integers(side_min, side_max + 1))
+        q = float(rng.uniform(500.0, 5000.0))
+        nm = float(rng.uniform(0.02, 0.04))
+        tide = float(rng.uniform(0.0, 1.0))
+        si, sj = N / 980.0, M / 1048.0
+        c = river_case(N=N, M=M, hours=hours, amp=150.0 * min(1.0, sj), wavelength=400.0 * si,
+                       n_islands=max(4, int(round(20 * si * sj))), n_manning=nm, q_mean=q, q_amp=0.0,
+                       tide_amp=tide, seed=1000 + k, sediment=sediment, name=f"ens{k:02d}_{N}x{M}")
+        if sediment:
+            c.sediment_start = 0.0
+            c.bed_change_start = 1800.0
+        cases.append(c)
+    return cases

@@ -1,6 +1,8 @@
 """GPU parity: libhydist (hand-written gfx950 kernels, through the C ABI)
 against the CPU checker, bit for bit, on seeded synthetic cases small enough
 for the checker to finish in seconds; plus error paths and run == step."""
+import os
+
 import numpy as np
 import pytest
 
@@ -142,3 +144,45 @@ def test_analytic_channel_boundaries():
     o.run(0.0, 10, op)
     g.run(0.0, 10, op)
     assert_same_state(o, g, "analytic channel mode")
+
+
+def test_long_runs_bitwise():
+    """Runs of up to 297 cells (the 300 x 60 channel's UZ lines): the packed
+    run solver's multi-chunk path (254 rows per chunk) against the checker."""
+    case = synth.channel_case(N=300, M=60, nsteps=40, name="channel_300x60")
+    o, g = pair(case)
+    op = case.opts()
+    t = 0.0
+    for k in range(20):
+        tn = o.step(t, op)
+        g.step(t, op)
+        t = tn
+    assert_same_state(o, g, "channel 300x60, 20 steps")
+
+
+def test_full_river_bitwise():
+    """The 980 x 1048 river itself, a few steps, against the checker."""
+    oracle.set_threads(min(16, len(os.sched_getaffinity(0))))
+    case = synth.config(2)
+    o, g = pair(case)
+    op = case.opts()
+    o.run(0.0, 4, op)
+    g.run(0.0, 4, op)
+    assert_same_state(o, g, "river 980x1048, 4 steps")
+
+
+@pytest.mark.parametrize("form", ["line", "rows"])
+def test_solver_forms_identical(form, monkeypatch):
+    """The three hydrodynamic solver forms (one wave per run, one wave per
+    line, rows kernel + lane per line) give identical bits."""
+    case = small_river()
+    ref = Solver(case)
+    monkeypatch.setenv("HYDIST_SOLVER", form)
+    alt = Solver(case)
+    monkeypatch.delenv("HYDIST_SOLVER")
+    ref.init()
+    alt.init()
+    op = case.opts()
+    ref.run(0.0, 30, op)
+    alt.run(0.0, 30, op)
+    assert_same_state(ref, alt, f"run-packed vs {form}")
