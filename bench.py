@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 B_STEP = 256               # algorithmic bytes per wet-cell-step (hydrodynamics)
+B_STEP_SED = 304           # ... with suspended sediment
 
 # Algorithmic bytes per wet cell of the per-launch kernels (compulsory FP64
 # reads/writes of the kernel's inputs and outputs, each element counted once).
@@ -52,6 +53,8 @@ def parse():
     p.add_argument("--profile-steps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--ensemble-steps", type=int, default=720)
+    p.add_argument("--no-ensemble", action="store_true")
     return p.parse_args()
 
 
@@ -87,6 +90,49 @@ def cpu_baseline(case, budget_s, threads):
             break
     o.close()
     return wet / dt, n, dt, used
+
+
+def ensemble_bench(nsteps):
+    """The batched-ensemble workload: 64 seeded meandering rivers (sides
+    256-1024, own discharge, roughness, tide; sediment on), one launch per
+    kernel over all of them, nsteps steps (720 = one simulated hour)."""
+    import numpy as np
+    from hydist_amd import synth
+    from hydist_amd._lib import Ensemble, Solver
+
+    t0 = time.perf_counter()
+    cases = synth.ensemble_cases()
+    ms = []
+    for c in cases:
+        g = Solver(c, device=0)
+        g.init()
+        ms.append(g)
+    setup = time.perf_counter() - t0
+    op = cases[0].opts()
+    ens = Ensemble(ms)
+    t = ens.run(0.0, 2, op)                      # capture the step graphs
+    for m in ms:
+        m.sync()
+    wc0 = sum(m.wet_count() for m in ms)
+    ens.reset_wet_counter()
+    t1 = time.perf_counter()
+    t = ens.run(t, nsteps, op, counted=True)
+    el = time.perf_counter() - t1
+    acc = ens.wet_cell_steps()
+    wc1 = sum(m.wet_count() for m in ms)
+    wet_steps = acc + wc0 - wc1
+    finite = sum(int(np.isfinite(m.get("z")).all() and np.isfinite(m.get("u")).all()) for m in ms)
+    cells = sum((c.N + 3) * (c.M + 3) for c in cases)
+    rate = wet_steps / el
+    out = {"workload": f"ensemble_{len(cases)}_rivers_256-1024", "scenarios": len(cases), "padded_cells": cells,
+           "steps": nsteps, "sim_hours": nsteps * cases[0].dT / 3600.0, "value": rate,
+           "unit": "wet-cell-steps/s", "ms_per_step": el * 1e3 / nsteps,
+           "mean_wet_cells": wet_steps / nsteps, "members_finite": finite,
+           "hbm_frac_step_algorithmic": rate * B_STEP_SED / HBM_PEAK, "setup_s": round(setup, 1)}
+    ens.close()
+    for m in ms:
+        m.close()
+    return out
 
 
 def main():
@@ -150,6 +196,8 @@ def main():
                "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
                          f"(FP64 restatement, OpenMP over lines, {used} threads)"}
 
+    ens = None if a.no_ensemble else ensemble_bench(a.ensemble_steps)
+
     line = {
         "metric": "ADI wet-cell-steps/s (FP64)",
         "value": value,
@@ -169,6 +217,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_us": kernels,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 

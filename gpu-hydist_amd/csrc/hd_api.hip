@@ -5,10 +5,12 @@
  */
 #include "hd_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <tuple>
@@ -16,7 +18,11 @@
 
 namespace {
 
-__global__ void k_set_t(double* tdev, double t) { tdev[0] = t; }
+/* set the device time of every context of a batch (grid.z = batch) */
+__global__ void k_set_t(const HdDev* dv, double t)
+{
+    if (threadIdx.x == 0) dv[blockIdx.z].tdev[0] = t;
+}
 
 struct Field {
     double** dptr;   /* points into hd_ctx::d so buffer swaps are seen */
@@ -67,31 +73,9 @@ const char* kNames[] = {"h", "hsnham", "VISCOIDX", "u", "v", "z", "t_u", "t_v", 
 
 } // namespace
 
-struct hd_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int N = 0, M = 0, SL = 0, series_len = 0, total_time = 0;
-    hd_params P;
-    HdDev d;
-    std::map<std::string, Field> fields;
-    std::vector<void*> allocs;
-    double* tdev = nullptr;
-    int* cand_cnt = nullptr;
-    int* cand = nullptr;
-    unsigned char* dryflag = nullptr;
-    void* arg_mirror = nullptr;
-    void* arr_mirror = nullptr;
-    bool initialised = false;
-    std::string err;
-    std::map<GraphKey, hipGraphExec_t> graphs;
-    /* hydrodynamic solver form (HYDIST_SOLVER): 0 = one wave per wet run
-     * (default), 1 = one wave per line ("line"), 2 = rows kernel + one lane
-     * per line ("rows").  All three give identical bits. */
-    int solver = 0;
-    bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
-    int run_blocks_u = 0, run_blocks_v = 0;
-    /* per-kernel HIP-event timing (hd_profile / hd_kernel_times) */
-    bool profile = false;
+/* per-kernel HIP-event timing (hd_profile / hd_kernel_times) */
+struct Prof {
+    bool on = false;
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
@@ -99,11 +83,83 @@ struct hd_ctx {
     std::map<std::string, std::pair<double, int>> acc;
 };
 
-static hipEvent_t take_event(hd_ctx* c)
+struct hd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0, M = 0, SL = 0, series_len = 0, total_time = 0;
+    hd_params P;
+    HdDev d;                    /* host copy; t_u/tmp_u and t_v/tmp_v swap with the smoothing passes */
+    double *t_u0 = nullptr, *t_v0 = nullptr; /* the buffers named t_u / t_v at creation */
+    HdDev* dv = nullptr;        /* device copies of d in the four swap states (bit 0: u, bit 1: v) */
+    std::map<std::string, Field> fields;
+    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> state_allocs;
+    void* arg_mirror = nullptr;
+    void* arr_mirror = nullptr;
+    bool initialised = false;
+    std::string err;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::map<GraphKey, hipGraphExec_t> sed_graphs;   /* sediment-only steps (hd_sediment_run) */
+    /* hydrodynamic solver form (HYDIST_SOLVER): 0 = one wave per wet run
+     * (default), 1 = one wave per line ("line"), 2 = rows kernel + one lane
+     * per line ("rows").  All three give identical bits. */
+    int solver = 0;
+    bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
+    int run_blocks_u = 0, run_blocks_v = 0;
+    int rb_mult = 2;            /* solver blocks per line (autotuned) */
+    int bw = 256;               /* block width of the per-row full-grid kernels (autotuned) */
+    Prof prof;
+    int state() const { return (d.t_u != t_u0 ? 1 : 0) | (d.t_v != t_v0 ? 2 : 0); }
+};
+
+/* Where a step is enqueued: one context (n = 1) or an ensemble of contexts
+ * on one device.  dv holds n device HdDev copies per swap state, state-major;
+ * a launch passes the copies of the current state. */
+struct Exec {
+    hipStream_t stream;
+    const HdDev* dv;
+    int n, N, M;
+    int state;
+    int solver;
+    bool solver_v1;
+    int run_blocks_u, run_blocks_v;
+    int bw;
+    Prof* prof;
+    hd_ctx* single;   /* mirrors the swaps into the context's host copy */
+    HdBatch b() const { return HdBatch{dv + (size_t)state * n, n, N, M}; }
+    void swap_u()
+    {
+        state ^= 1;
+        if (single) std::swap(single->d.t_u, single->d.tmp_u);
+    }
+    void swap_v()
+    {
+        state ^= 2;
+        if (single) std::swap(single->d.t_v, single->d.tmp_v);
+    }
+};
+
+/* solver grid: rb_mult blocks per line, at most one per table entry */
+static void set_knobs(hd_ctx* c, int rb_mult, int bw)
 {
-    if (!c->pool.empty()) {
-        hipEvent_t e = c->pool.back();
-        c->pool.pop_back();
+    int N = c->N, M = c->M, SL = c->SL;
+    c->rb_mult = rb_mult;
+    c->bw = bw;
+    c->run_blocks_u = rb_mult * (M + 3) < (M + 3) * SL ? rb_mult * (M + 3) : (M + 3) * SL;
+    c->run_blocks_v = rb_mult * (N + 3) < (N + 3) * SL ? rb_mult * (N + 3) : (N + 3) * SL;
+}
+
+static Exec exec_of(hd_ctx* c)
+{
+    return Exec{c->stream, c->dv, 1, c->N, c->M, c->state(), c->solver, c->solver_v1, c->run_blocks_u,
+                c->run_blocks_v, c->bw, &c->prof, c};
+}
+
+static hipEvent_t take_event(Prof* p)
+{
+    if (!p->pool.empty()) {
+        hipEvent_t e = p->pool.back();
+        p->pool.pop_back();
         return e;
     }
     hipEvent_t e = nullptr;
@@ -111,17 +167,16 @@ static hipEvent_t take_event(hd_ctx* c)
     return e;
 }
 
-/* Run one launcher, bracketed by events on the context's stream when
- * profiling is on. */
+/* Run one launcher, bracketed by events on the stream when profiling is on. */
 template <class F>
-static void launch(hd_ctx* c, const char* name, F&& f)
+static void launch(Exec& x, const char* name, F&& f)
 {
-    if (!c->profile) { f(); return; }
-    hd_ctx::Rec r{name, take_event(c), take_event(c)};
-    (void)hipEventRecord(r.a, c->stream);
+    if (!x.prof || !x.prof->on) { f(); return; }
+    Prof::Rec r{name, take_event(x.prof), take_event(x.prof)};
+    (void)hipEventRecord(r.a, x.stream);
     f();
-    (void)hipEventRecord(r.b, c->stream);
-    c->pending.push_back(r);
+    (void)hipEventRecord(r.b, x.stream);
+    x.prof->pending.push_back(r);
 }
 
 static thread_local std::string g_err;
@@ -151,15 +206,27 @@ static void derive_params(hd_params& p)
     p.wss = 10 * p.muy / p.dm * (std::sqrt(1 + 0.01 * (p.Sx - 1) * 9.81 * hd_pow(p.dm, 3) / hd_pow(p.muy, 2)) - 1);
 }
 
-static int dalloc(hd_ctx* c, void** p, size_t bytes)
+/* device allocation of a context, zero-filled; `state` allocations hold
+ * data (copied by hd_clone), the others hold pointer tables */
+static int dalloc(hd_ctx* c, void** p, size_t bytes, bool state = true)
 {
     if (bytes == 0) bytes = 8;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return fail(c, HD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->allocs.push_back(*p);
+    if (state) c->state_allocs.push_back(std::make_pair(*p, bytes));
     e = hipMemset(*p, 0, bytes);
     if (e != hipSuccess) return fail(c, HD_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
     return HD_OK;
+}
+
+/* Device HdDev of a context in its current swap state, its stream and the
+ * state (used by the surrogate's packing kernels in hd_cnn.hip). */
+const HdDev* hd_ctx_device_view(hd_ctx* c, hipStream_t* stream, int* state)
+{
+    if (stream) *stream = c->stream;
+    if (state) *state = c->state();
+    return c->dv + c->state();
 }
 
 extern "C" {
@@ -259,17 +326,31 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     if ((rc = dalloc(c, (void**)&d.run_cnt, 4 * 4))) { hd_destroy(c); return rc; }
     /* one block per run while the table holds up to twice as many runs as
        lines; blocks stride over the rest */
-    c->run_blocks_u = 2 * (M + 3) < (M + 3) * SL ? 2 * (M + 3) : (M + 3) * SL;
-    c->run_blocks_v = 2 * (N + 3) < (N + 3) * SL ? 2 * (N + 3) : (N + 3) * SL;
+    set_knobs(c, c->rb_mult, c->bw);
     if ((rc = dalloc(c, (void**)&d.flags, 16 * 4))) { hd_destroy(c); return rc; }
     /* [0]: per-step accumulator of hd_run_counted, [1]: one-shot count of hd_wet_count */
     if ((rc = dalloc(c, (void**)&d.wet_count, 16))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, (void**)&c->tdev, 8))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, (void**)&c->cand_cnt, nlines * 4))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, (void**)&c->cand, nlines * 64 * 4))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, (void**)&c->dryflag, F))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, &c->arg_mirror, sizeof(ArgMirror)))) { hd_destroy(c); return rc; }
-    if ((rc = dalloc(c, &c->arr_mirror, sizeof(ArrMirror)))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.tdev, 8))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.cand_cnt, nlines * 4))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.cand, nlines * 64 * 4))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, (void**)&d.dryflag, F))) { hd_destroy(c); return rc; }
+    c->t_u0 = d.t_u;
+    c->t_v0 = d.t_v;
+    if ((rc = dalloc(c, (void**)&c->dv, 4 * sizeof(HdDev), false))) { hd_destroy(c); return rc; }
+    {
+        HdDev st[4];
+        for (int k = 0; k < 4; ++k) {
+            st[k] = d;
+            if (k & 1) std::swap(st[k].t_u, st[k].tmp_u);
+            if (k & 2) std::swap(st[k].t_v, st[k].tmp_v);
+        }
+        if (hipMemcpy(c->dv, st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess) {
+            hd_destroy(c);
+            return fail(nullptr, HD_EHIP, "hipMemcpy device state copies");
+        }
+    }
+    if ((rc = dalloc(c, &c->arg_mirror, sizeof(ArgMirror), false))) { hd_destroy(c); return rc; }
+    if ((rc = dalloc(c, &c->arr_mirror, sizeof(ArrMirror), false))) { hd_destroy(c); return rc; }
     /* the reference's host arrays start as zeros except khouot = 2
        (Global_Variables.py:93-127) */
     std::vector<int> two(F, 2);
@@ -287,8 +368,9 @@ int hd_destroy(hd_ctx* c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
-    for (auto& r : c->pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-    for (auto e : c->pool) hipEventDestroy(e);
+    for (auto& g : c->sed_graphs) hipGraphExecDestroy(g.second);
+    for (auto& r : c->prof.pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto e : c->prof.pool) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -377,11 +459,11 @@ int hd_init(hd_ctx* c)
 {
     if (!c) return HD_EINVAL;
     HIPCHK(c, hipSetDevice(c->device));
-    const HdDev& d = c->d;
-    hd_launch_onetime_init(d, c->stream);
-    hd_launch_find_limits(d, c->stream);
-    hd_launch_htuongdoi(d, c->stream);
-    hd_launch_preprocess(d, c->stream);
+    HdBatch b = exec_of(c).b();
+    hd_launch_onetime_init(b, c->stream);
+    hd_launch_find_limits(b, c->stream);
+    hd_launch_htuongdoi(b, c->stream);
+    hd_launch_preprocess(b, c->stream);
     HIPCHK(c, hipGetLastError());
     int rc = check_flags(c);
     if (rc) return rc;
@@ -397,9 +479,10 @@ int hd_prepare(hd_ctx* c)
     if (!c) return HD_EINVAL;
     if (!c->initialised) return fail(c, HD_ESTATE, "hd_prepare before hd_init");
     HIPCHK(c, hipSetDevice(c->device));
-    hd_launch_find_limits(c->d, c->stream);
-    hd_launch_htuongdoi(c->d, c->stream);
-    hd_launch_preprocess(c->d, c->stream);
+    HdBatch b = exec_of(c).b();
+    hd_launch_find_limits(b, c->stream);
+    hd_launch_htuongdoi(b, c->stream);
+    hd_launch_preprocess(b, c->stream);
     HIPCHK(c, hipGetLastError());
     return check_flags(c);
 }
@@ -428,90 +511,85 @@ static int check_series(hd_ctx* c, double t, const hd_step_opts& o)
     return HD_OK;
 }
 
-static void swap_u(hd_ctx* c) { std::swap(c->d.t_u, c->d.tmp_u); }
-static void swap_v(hd_ctx* c) { std::swap(c->d.t_v, c->d.tmp_v); }
-
 /* One half-step of Engine.py:146-239 (which = 0, UZ) or :260-361 (which = 1,
- * VZ), enqueued on the stream.  The device time is advanced by the boundary
- * kernel. */
-static void enqueue_half(hd_ctx* c, int which, const hd_step_opts& o, bool sediment, int count_wet, int* sidx_out,
-                         int* eidx_out)
+ * VZ), enqueued on the stream for every context of the batch.  The device
+ * time is advanced by the boundary kernel.  Returns in *slo / *eoff the row
+ * range of a following bed change (rows slo .. N - eoff - 1). */
+static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sediment, int count_wet, int* slo_out,
+                         int* eoff_out)
 {
-    hipStream_t s = c->stream;
-    int N = c->N, M = c->M;
+    hipStream_t s = x.stream;
     bool channel = o.kenhhepd || o.kenhhepng;
     int an = analytic_mode(o) ? 1 : 0;
-    launch(c, "boundary", [&] { hd_launch_boundary_t(c->d, c->tdev, an, s); });
-    int sidx, eidx;
+    launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, s); });
+    int slo = 2, eoff = 0;
     if (which == 0) {
-        sidx = 2; eidx = M;
-        if (channel && o.kenhhepd == 1) { sidx = 3; eidx = M - 1; }
-        if (c->solver == 2) {
-            launch(c, "adi_coef_uz", [&] { hd_launch_adi_coef(c->d, 1, sidx, eidx, s); });
-            launch(c, "adi_solve_uz", [&] { hd_launch_adi_solve(c->d, 1, sidx, eidx, s); });
-        } else if (c->solver == 1) {
-            launch(c, "adi_line_uz", [&] { hd_launch_adi_line(c->d, 1, sidx, eidx, s); });
+        int shift = (channel && o.kenhhepd == 1) ? 1 : 0;
+        if (x.solver == 2) {
+            launch(x, "adi_coef_uz", [&] { hd_launch_adi_coef(x.b(), 1, shift, s); });
+            launch(x, "adi_solve_uz", [&] { hd_launch_adi_solve(x.b(), 1, shift, s); });
+        } else if (x.solver == 1) {
+            launch(x, "adi_line_uz", [&] { hd_launch_adi_line(x.b(), 1, shift, s); });
         } else {
-            launch(c, "adi_run_uz", [&] { hd_launch_adi_run(c->d, 1, sidx, eidx, c->run_blocks_u, s); });
+            launch(x, "adi_run_uz", [&] { hd_launch_adi_run(x.b(), 1, shift, x.run_blocks_u, s); });
         }
-        launch(c, "smooth_u", [&] { hd_launch_normalize_post(c->d, 1, 0, 1, c->cand_cnt, c->cand, c->dryflag, s); });
-        swap_u(c);
-        launch(c, "explicit_v", [&] { hd_launch_explicit(c->d, 0, 2, N, s); });
-        launch(c, "margins_v", [&] { hd_launch_margins(c->d, 0, 2, N, s); });
-        launch(c, "smooth_v_hmoi", [&] { hd_launch_normalize_post(c->d, 0, 1, 1, c->cand_cnt, c->cand, c->dryflag, s); });
-        swap_v(c);
-        launch(c, "wetting_rows", [&] { hd_launch_wetting(c->d, 1, c->cand_cnt, c->cand, c->dryflag, s); });
+        launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, s); });
+        x.swap_u();
+        launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
+        launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
+        launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, s); });
+        x.swap_v();
+        launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, s); });
     } else {
-        sidx = 2; eidx = N;
-        if (channel && o.kenhhepng == 1) { sidx = 3; eidx = N; }
-        if (c->solver == 2) {
-            launch(c, "adi_coef_vz", [&] { hd_launch_adi_coef(c->d, 0, sidx, eidx, s); });
-            launch(c, "adi_solve_vz", [&] { hd_launch_adi_solve(c->d, 0, sidx, eidx, s); });
-        } else if (c->solver == 1) {
-            launch(c, "adi_line_vz", [&] { hd_launch_adi_line(c->d, 0, sidx, eidx, s); });
+        int shift = (channel && o.kenhhepng == 1) ? 1 : 0;
+        slo = 2 + shift;
+        if (x.solver == 2) {
+            launch(x, "adi_coef_vz", [&] { hd_launch_adi_coef(x.b(), 0, shift, s); });
+            launch(x, "adi_solve_vz", [&] { hd_launch_adi_solve(x.b(), 0, shift, s); });
+        } else if (x.solver == 1) {
+            launch(x, "adi_line_vz", [&] { hd_launch_adi_line(x.b(), 0, shift, s); });
         } else {
-            launch(c, "adi_run_vz", [&] { hd_launch_adi_run(c->d, 0, sidx, eidx, c->run_blocks_v, s); });
+            launch(x, "adi_run_vz", [&] { hd_launch_adi_run(x.b(), 0, shift, x.run_blocks_v, s); });
         }
-        launch(c, "smooth_v", [&] { hd_launch_normalize_post(c->d, 0, 0, 0, c->cand_cnt, c->cand, c->dryflag, s); });
-        swap_v(c);
-        launch(c, "explicit_u", [&] { hd_launch_explicit(c->d, 1, 2, M, s); });
-        launch(c, "margins_u", [&] { hd_launch_margins(c->d, 1, 2, M, s); });
-        launch(c, "smooth_u_hmoi", [&] { hd_launch_normalize_post(c->d, 1, 1, 0, c->cand_cnt, c->cand, c->dryflag, s); });
-        swap_u(c);
-        launch(c, "wetting_cols", [&] { hd_launch_wetting(c->d, 0, c->cand_cnt, c->cand, c->dryflag, s); });
+        launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, s); });
+        x.swap_v();
+        launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
+        launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
+        launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, s); });
+        x.swap_u();
+        launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
     }
-    launch(c, "uvz_dry_htd", [&] { hd_launch_uvz_dry(c->d, o.kenhhepd, o.kenhhepng, c->dryflag, s); });
-    launch(c, "segments", [&] { hd_launch_segments(c->d, count_wet, s); });
-    launch(c, "fs_edges", [&] { hd_launch_fs_edges(c->d, s); });
+    launch(x, "uvz_dry_htd", [&] { hd_launch_uvz_dry(x.b(), o.kenhhepd, o.kenhhepng, x.bw, s); });
+    launch(x, "segments_fs_edges", [&] { hd_launch_segments(x.b(), count_wet, s); });
     if (sediment) {
-        launch(c, "vth_hesok", [&] { hd_launch_vth_hesok(c->d, s); });
-        if (c->solver_v1) {
-            launch(c, which == 0 ? "fs_coef_j" : "fs_coef_i",
-                   [&] { hd_launch_fs_coef(c->d, which, c->tdev, o.bed_change_start, o.ketdinh, s); });
-            launch(c, which == 0 ? "fs_solve_j" : "fs_solve_i", [&] { hd_launch_fs_solve(c->d, which, s); });
+        launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), s); });
+        if (x.solver_v1) {
+            launch(x, which == 0 ? "fs_coef_j" : "fs_coef_i",
+                   [&] { hd_launch_fs_coef(x.b(), which, o.bed_change_start, o.ketdinh, s); });
+            launch(x, which == 0 ? "fs_solve_j" : "fs_solve_i", [&] { hd_launch_fs_solve(x.b(), which, s); });
         } else {
-            launch(c, which == 0 ? "fs_line_j" : "fs_line_i",
-                   [&] { hd_launch_fs_line(c->d, which, c->tdev, o.bed_change_start, o.ketdinh, s); });
+            launch(x, which == 0 ? "fs_line_j" : "fs_line_i",
+                   [&] { hd_launch_fs_line(x.b(), which, o.bed_change_start, o.ketdinh, s); });
         }
-        launch(c, "update_fs", [&] { hd_launch_update_fs(c->d, s); });
-        if (which == 1) { sidx = 3; eidx = N - 1; }
+        launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), s); });
+        if (which == 1) { slo = 3; eoff = 1; }
     }
-    if (sidx_out) *sidx_out = sidx;
-    if (eidx_out) *eidx_out = eidx;
+    if (slo_out) *slo_out = slo;
+    if (eoff_out) *eoff_out = eoff;
 }
 
-static void enqueue_bed(hd_ctx* c, const hd_step_opts& o, int sidx, int eidx)
+static void enqueue_bed(Exec& x, const hd_step_opts& o, int slo, int eoff)
 {
-    launch(c, "calc_qb", [&] { hd_launch_calc_qb(c->d, o.qb_mode, c->stream); });
-    launch(c, "bedload", [&] { hd_launch_bedload(c->d, o.ketdinh, sidx, eidx, c->stream); });
+    launch(x, "calc_qb", [&] { hd_launch_calc_qb(x.b(), o.qb_mode, x.stream); });
+    launch(x, "bedload", [&] { hd_launch_bedload(x.b(), o.ketdinh, slo, eoff, x.stream); });
 }
 
-static void enqueue_step(hd_ctx* c, const hd_step_opts& o, const GraphKey& k)
+static void enqueue_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
 {
-    int sidx = 0, eidx = 0;
-    enqueue_half(c, 0, o, k.sed0, 0, nullptr, nullptr);
-    enqueue_half(c, 1, o, k.sed1, k.count, &sidx, &eidx);
-    if (k.bed) enqueue_bed(c, o, sidx, eidx);
+    int slo = 2, eoff = 0;
+    enqueue_half(x, 0, o, k.sed0, 0, nullptr, nullptr);
+    enqueue_half(x, 1, o, k.sed1, k.count, &slo, &eoff);
+    if (k.bed) enqueue_bed(x, o, slo, eoff);
 }
 
 static GraphKey make_key(hd_ctx* c, double t, const hd_step_opts& o, int count)
@@ -530,6 +608,11 @@ static GraphKey make_key(hd_ctx* c, double t, const hd_step_opts& o, int count)
     return k;
 }
 
+static void set_time(const Exec& x, double t)
+{
+    hipLaunchKernelGGL(k_set_t, dim3(1, 1, x.n), dim3(64), 0, x.stream, x.dv, t);
+}
+
 int hd_step(hd_ctx* c, double t, const hd_step_opts* o, double* t_out)
 {
     if (!c || !o) return HD_EINVAL;
@@ -538,9 +621,10 @@ int hd_step(hd_ctx* c, double t, const hd_step_opts* o, double* t_out)
     double t1 = t + c->P.dT * 0.5, t2 = t1 + c->P.dT * 0.5;
     int rc;
     if ((rc = check_series(c, t1, *o)) || (rc = check_series(c, t2, *o))) return rc;
-    hipLaunchKernelGGL(k_set_t, dim3(1), dim3(1), 0, c->stream, c->tdev, t);
+    Exec x = exec_of(c);
+    set_time(x, t);
     GraphKey k = make_key(c, t, *o, 0);
-    enqueue_step(c, *o, k);
+    enqueue_step(x, *o, k);
     HIPCHK(c, hipGetLastError());
     if (t_out) *t_out = t2;
     return HD_OK;
@@ -553,8 +637,9 @@ int hd_halfstep(hd_ctx* c, int which, double t, const hd_step_opts* o)
     HIPCHK(c, hipSetDevice(c->device));
     int rc = check_series(c, t, *o);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_set_t, dim3(1), dim3(1), 0, c->stream, c->tdev, t - c->P.dT * 0.5);
-    enqueue_half(c, which, *o, t >= o->sediment_start, 0, nullptr, nullptr);
+    Exec x = exec_of(c);
+    set_time(x, t - c->P.dT * 0.5);
+    enqueue_half(x, which, *o, t >= o->sediment_start, 0, nullptr, nullptr);
     HIPCHK(c, hipGetLastError());
     return HD_OK;
 }
@@ -562,13 +647,80 @@ int hd_halfstep(hd_ctx* c, int which, double t, const hd_step_opts* o)
 int hd_bed_change(hd_ctx* c, double t, const hd_step_opts* o)
 {
     if (!c || !o) return HD_EINVAL;
-    (void)t;
-    int sidx = 2, eidx = c->N;
-    if ((o->kenhhepd || o->kenhhepng) && o->kenhhepng == 1) sidx = 3;
-    if (t >= o->sediment_start) { sidx = 3; eidx = c->N - 1; }
-    enqueue_bed(c, *o, sidx, eidx);
+    int slo = 2, eoff = 0;
+    if ((o->kenhhepd || o->kenhhepng) && o->kenhhepng == 1) slo = 3;
+    if (t >= o->sediment_start) { slo = 3; eoff = 1; }
+    Exec x = exec_of(c);
+    enqueue_bed(x, *o, slo, eoff);
     HIPCHK(c, hipGetLastError());
     return HD_OK;
+}
+
+/* Enqueue nsteps full steps from t: the first time a step kind (trigger
+ * pattern) occurs it is captured into a HIP graph, later steps replay it.
+ * A full step swaps each smoothing pair twice, so the replay's pointers
+ * (the state-0 device copies) hold.  Returns 0 or the HIP error. */
+typedef void (*EnqueueFn)(Exec&, const hd_step_opts&, const GraphKey&);
+
+static hipError_t run_steps(Exec& x, std::map<GraphKey, hipGraphExec_t>& graphs, double& t, int nsteps,
+                            const hd_step_opts& o, double dT, int count, const std::function<GraphKey(double)>& key,
+                            std::string& why, EnqueueFn enq = enqueue_step)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        GraphKey k = key(t);
+        double t2 = t + dT * 0.5 + dT * 0.5;
+        if (x.prof && x.prof->on) {
+            enq(x, o, k);
+            t = t2;
+            continue;
+        }
+        auto it = graphs.find(k);
+        if (it == graphs.end()) {
+            int st0 = x.state;
+            hipGraph_t g;
+            hipError_t e = hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) { why = "begin capture"; return e; }
+            enq(x, o, k);
+            e = hipStreamEndCapture(x.stream, &g);
+            if (e != hipSuccess) { why = "graph capture"; return e; }
+            hipGraphExec_t ge;
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) { why = "graph instantiate"; return e; }
+            if (x.state != st0) {
+                hipGraphExecDestroy(ge);
+                why = "step changed the buffer assignment";
+                return hipErrorUnknown;
+            }
+            it = graphs.emplace(k, ge).first;
+        }
+        hipError_t e = hipGraphLaunch(it->second, x.stream);
+        if (e != hipSuccess) { why = "graph launch"; return e; }
+        t = t2;
+    }
+    (void)count;
+    return hipSuccess;
+}
+
+/* The sediment half of one step on a frozen flow: per half-step the
+ * boundary values (device time advanced) and, when the sediment module is on,
+ * VTH / hesoK, the FS sweep of that direction and the FS update; then the
+ * bed change when triggered. */
+static void enqueue_sed_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
+{
+    int an = analytic_mode(o) ? 1 : 0;
+    int slo = 2, eoff = 0;
+    for (int which = 0; which < 2; ++which) {
+        launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, x.stream); });
+        if (which == 0 ? k.sed0 : k.sed1) {
+            launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), x.stream); });
+            launch(x, which == 0 ? "fs_line_j" : "fs_line_i",
+                   [&] { hd_launch_fs_line(x.b(), which, o.bed_change_start, o.ketdinh, x.stream); });
+            launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), x.stream); });
+            if (which == 1) { slo = 3; eoff = 1; }
+        }
+    }
+    if (k.bed) enqueue_bed(x, o, slo, eoff);
 }
 
 static int run_impl(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out, int count)
@@ -576,40 +728,18 @@ static int run_impl(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, doub
     if (!c || !o || nsteps < 0) return HD_EINVAL;
     if (!c->initialised) return fail(c, HD_ESTATE, "hd_run before hd_init");
     HIPCHK(c, hipSetDevice(c->device));
-    hipLaunchKernelGGL(k_set_t, dim3(1), dim3(1), 0, c->stream, c->tdev, t);
-    for (int n = 0; n < nsteps; ++n) {
-        double t1 = t + c->P.dT * 0.5, t2 = t1 + c->P.dT * 0.5;
+    for (int n = 0; n < nsteps; ++n) {  /* series long enough for every half-step */
+        double tn = t + n * c->P.dT;
         int rc;
-        if ((rc = check_series(c, t1, *o)) || (rc = check_series(c, t2, *o))) return rc;
-        GraphKey k = make_key(c, t, *o, count);
-        if (c->profile) {
-            enqueue_step(c, *o, k);
-            HIPCHK(c, hipGetLastError());
-            t = t2;
-            continue;
-        }
-        auto it = c->graphs.find(k);
-        if (it == c->graphs.end()) {
-            HdDev saved = c->d;
-            hipGraph_t g;
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c, *o, k);
-            hipError_t e = hipStreamEndCapture(c->stream, &g);
-            if (e != hipSuccess) return fail(c, HD_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-            hipGraphExec_t ge;
-            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(c, HD_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-            /* a full step swaps each smoothing pair twice: pointers are back */
-            if (std::memcmp(&saved, &c->d, sizeof(HdDev)) != 0) {
-                hipGraphExecDestroy(ge);
-                return fail(c, HD_EHIP, "step changed the buffer assignment");
-            }
-            it = c->graphs.emplace(k, ge).first;
-        }
-        HIPCHK(c, hipGraphLaunch(it->second, c->stream));
-        t = t2;
+        if ((rc = check_series(c, tn + c->P.dT * 0.5, *o)) || (rc = check_series(c, tn + c->P.dT, *o))) return rc;
     }
+    Exec x = exec_of(c);
+    set_time(x, t);
+    std::string why;
+    hipError_t e = run_steps(x, c->graphs, t, nsteps, *o, c->P.dT, count,
+                             [&](double tt) { return make_key(c, tt, *o, count); }, why);
+    if (e != hipSuccess) return fail(c, HD_EHIP, why + ": " + hipGetErrorString(e));
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (t_out) *t_out = t;
     return check_flags(c);
@@ -626,6 +756,128 @@ int hd_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out
 int hd_run_counted(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out)
 {
     return run_impl(c, t, nsteps, o, t_out, 1);
+}
+
+int hd_sediment_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out)
+{
+    if (!c || !o || nsteps < 0) return HD_EINVAL;
+    if (!c->initialised) return fail(c, HD_ESTATE, "hd_sediment_run before hd_init");
+    if (c->state() != 0) return fail(c, HD_ESTATE, "hd_sediment_run between half-steps");
+    HIPCHK(c, hipSetDevice(c->device));
+    for (int n = 0; n < nsteps; ++n) {
+        double tn = t + n * c->P.dT;
+        int rc;
+        if ((rc = check_series(c, tn + c->P.dT * 0.5, *o)) || (rc = check_series(c, tn + c->P.dT, *o))) return rc;
+    }
+    Exec x = exec_of(c);
+    set_time(x, t);
+    std::string why;
+    hipError_t e = run_steps(x, c->sed_graphs, t, nsteps, *o, c->P.dT, 0,
+                             [&](double tt) { return make_key(c, tt, *o, 0); }, why, enqueue_sed_step);
+    if (e != hipSuccess) return fail(c, HD_EHIP, why + ": " + hipGetErrorString(e));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t_out) *t_out = t;
+    return check_flags(c);
+}
+
+int hd_flow_refresh(hd_ctx* c)
+{
+    if (!c) return HD_EINVAL;
+    if (!c->initialised) return fail(c, HD_ESTATE, "hd_flow_refresh before hd_init");
+    if (c->state() != 0) return fail(c, HD_ESTATE, "hd_flow_refresh between half-steps");
+    HIPCHK(c, hipSetDevice(c->device));
+    hd_launch_flow_refresh(exec_of(c).b(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HD_OK;
+}
+
+int hd_clone(hd_ctx* src, hd_ctx** out)
+{
+    if (!src || !out) return HD_EINVAL;
+    if (!src->initialised) return fail(src, HD_ESTATE, "hd_clone before hd_init");
+    if (src->state() != 0) return fail(src, HD_ESTATE, "hd_clone between half-steps");
+    hd_params p = src->P;
+    int rc = hd_create(src->device, &p, src->N, src->M, src->series_len, src->total_time, out);
+    if (rc) return fail(src, rc, std::string("hd_clone: ") + g_err);
+    hd_ctx* c = *out;
+    HIPCHK(src, hipStreamSynchronize(src->stream));
+    if (c->state_allocs.size() != src->state_allocs.size()) { hd_destroy(c); return fail(src, HD_EHIP, "layout"); }
+    for (size_t k = 0; k < c->state_allocs.size(); ++k) {
+        if (c->state_allocs[k].second != src->state_allocs[k].second) { hd_destroy(c); return fail(src, HD_EHIP, "layout"); }
+        HIPCHK(src, hipMemcpyAsync(c->state_allocs[k].first, src->state_allocs[k].first, src->state_allocs[k].second,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(src, hipStreamSynchronize(c->stream));
+    set_knobs(c, src->rb_mult, src->bw);
+    c->initialised = true;
+    return HD_OK;
+}
+
+/* Time `steps` replayed steps of a context from t (events on its stream). */
+static int time_steps(hd_ctx* c, double t, const hd_step_opts* o, int steps, double* ms)
+{
+    int rc = run_impl(c, t, 2, o, nullptr, 0); /* capture */
+    if (rc) return rc;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    rc = run_impl(c, t + 2 * c->P.dT, steps, o, nullptr, 0);
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float f = 0;
+    HIPCHK(c, hipEventElapsedTime(&f, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = f / steps;
+    return rc;
+}
+
+int hd_autotune(hd_ctx* c, double t, const hd_step_opts* o, int steps, int* rb_mult, int* bw, double* ms_per_step)
+{
+    if (!c || !o || steps < 1) return HD_EINVAL;
+    static const int kRb[] = {1, 2, 4};
+    static const int kBw[] = {64, 128, 256};
+    double best = 1e30;
+    int brb = c->rb_mult, bbw = c->bw;
+    for (int r : kRb)
+        for (int w : kBw) {
+            hd_ctx* s = nullptr;
+            int rc = hd_clone(c, &s);
+            if (rc) return rc;
+            set_knobs(s, r, w);
+            double ms = 0;
+            rc = time_steps(s, t, o, steps, &ms);
+            if (rc) {
+                std::string m = s->err;
+                hd_destroy(s);
+                return fail(c, rc, "autotune: " + m);
+            }
+            hd_destroy(s);
+            if (ms < best) { best = ms; brb = r; bbw = w; }
+        }
+    for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    for (auto& g : c->sed_graphs) hipGraphExecDestroy(g.second);
+    c->sed_graphs.clear();
+    set_knobs(c, brb, bbw);
+    if (rb_mult) *rb_mult = brb;
+    if (bw) *bw = bbw;
+    if (ms_per_step) *ms_per_step = best;
+    return HD_OK;
+}
+
+int hd_set_launch(hd_ctx* c, int rb_mult, int bw)
+{
+    if (!c || rb_mult < 1 || rb_mult > 16 || (bw != 64 && bw != 128 && bw != 256)) return HD_EINVAL;
+    for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    for (auto& g : c->sed_graphs) hipGraphExecDestroy(g.second);
+    c->sed_graphs.clear();
+    set_knobs(c, rb_mult, bw);
+    return HD_OK;
 }
 
 int hd_reset_wet_counter(hd_ctx* c)
@@ -649,11 +901,10 @@ int hd_wet_count(hd_ctx* c, long long* out)
 {
     if (!c || !out) return HD_EINVAL;
     unsigned long long z = 0;
-    HdDev one = c->d;
-    one.wet_count = c->d.wet_count + 1;   /* leave the run accumulator alone */
-    HIPCHK(c, hipMemcpyAsync(one.wet_count, &z, 8, hipMemcpyHostToDevice, c->stream));
-    hd_launch_wet_count(one, c->stream);
-    HIPCHK(c, hipMemcpyAsync(out, one.wet_count, 8, hipMemcpyDeviceToHost, c->stream));
+    /* slot 1: the one-shot count (slot 0 is the run accumulator) */
+    HIPCHK(c, hipMemcpyAsync(c->d.wet_count + 1, &z, 8, hipMemcpyHostToDevice, c->stream));
+    hd_launch_wet_count(exec_of(c).b(), c->stream);
+    HIPCHK(c, hipMemcpyAsync(out, c->d.wet_count + 1, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return HD_OK;
 }
@@ -696,15 +947,47 @@ int hd_struct_ptrs(hd_ctx* c, void** arg, void** arr)
     return HD_OK;
 }
 
+static void prof_reset(Prof& p, bool on)
+{
+    for (auto& r : p.pending) { p.pool.push_back(r.a); p.pool.push_back(r.b); }
+    p.pending.clear();
+    p.acc.clear();
+    p.order.clear();
+    p.on = on;
+}
+
+static hipError_t prof_collect(Prof& p, int n, const char** names, double* total_ms, int* launches, int* count)
+{
+    for (auto& r : p.pending) {
+        float ms = 0.f;
+        hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
+        if (e != hipSuccess) return e;
+        auto it = p.acc.find(r.name);
+        if (it == p.acc.end()) { p.order.push_back(r.name); it = p.acc.emplace(r.name, std::make_pair(0.0, 0)).first; }
+        it->second.first += ms;
+        it->second.second += 1;
+        p.pool.push_back(r.a);
+        p.pool.push_back(r.b);
+    }
+    p.pending.clear();
+    int k = 0;
+    for (const std::string& nm : p.order) {
+        if (k >= n) break;
+        auto it = p.acc.find(nm);
+        if (names) names[k] = it->first.c_str();
+        if (total_ms) total_ms[k] = it->second.first;
+        if (launches) launches[k] = it->second.second;
+        ++k;
+    }
+    *count = k;
+    return hipSuccess;
+}
+
 int hd_profile(hd_ctx* c, int enable)
 {
     if (!c) return HD_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (auto& r : c->pending) { c->pool.push_back(r.a); c->pool.push_back(r.b); }
-    c->pending.clear();
-    c->acc.clear();
-    c->order.clear();
-    c->profile = enable != 0;
+    prof_reset(c->prof, enable != 0);
     return HD_OK;
 }
 
@@ -712,26 +995,141 @@ int hd_kernel_times(hd_ctx* c, int n, const char** names, double* total_ms, int*
 {
     if (!c) return HD_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (auto& r : c->pending) {
-        float ms = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&ms, r.a, r.b));
-        auto it = c->acc.find(r.name);
-        if (it == c->acc.end()) { c->order.push_back(r.name); it = c->acc.emplace(r.name, std::make_pair(0.0, 0)).first; }
-        it->second.first += ms;
-        it->second.second += 1;
-        c->pool.push_back(r.a);
-        c->pool.push_back(r.b);
-    }
-    c->pending.clear();
     int k = 0;
-    for (const std::string& nm : c->order) {
-        if (k >= n) break;
-        auto& a = c->acc[nm];
-        if (names) names[k] = c->acc.find(nm)->first.c_str();
-        if (total_ms) total_ms[k] = a.first;
-        if (launches) launches[k] = a.second;
-        ++k;
+    HIPCHK(c, prof_collect(c->prof, n, names, total_ms, launches, &k));
+    return k;
+}
+
+/* ================================================================ ensembles */
+
+struct hd_ensemble {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<hd_ctx*> m;
+    HdDev* dv = nullptr;   /* [4][n] device copies, state-major */
+    int N = 0, M = 0, run_blocks_u = 0, run_blocks_v = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    Prof prof;
+    std::string err;
+};
+
+static int efail(hd_ensemble* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    g_err = msg;
+    return code;
+}
+
+int hd_ensemble_create(hd_ctx* const* members, int n, hd_ensemble** out)
+{
+    if (!members || n < 1 || !out) return efail(nullptr, HD_EINVAL, "hd_ensemble_create: bad arguments");
+    hd_ctx* c0 = members[0];
+    for (int k = 0; k < n; ++k) {
+        hd_ctx* c = members[k];
+        if (!c || !c->initialised) return efail(nullptr, HD_ESTATE, "ensemble member not initialised");
+        if (c->device != c0->device || c->P.dT != c0->P.dT || c->solver != c0->solver ||
+            c->solver_v1 != c0->solver_v1)
+            return efail(nullptr, HD_EINVAL, "ensemble members must share device, dT and solver form");
+        if (c->state() != 0) return efail(nullptr, HD_ESTATE, "ensemble member between half-steps");
     }
+    if (hipSetDevice(c0->device) != hipSuccess) return efail(nullptr, HD_EHIP, "hipSetDevice");
+    hd_ensemble* e = new hd_ensemble();
+    e->device = c0->device;
+    e->m.assign(members, members + n);
+    for (hd_ctx* c : e->m) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) { delete e; return efail(nullptr, HD_EHIP, "sync"); }
+        e->N = std::max(e->N, c->N);
+        e->M = std::max(e->M, c->M);
+        e->run_blocks_u = std::max(e->run_blocks_u, c->run_blocks_u);
+        e->run_blocks_v = std::max(e->run_blocks_v, c->run_blocks_v);
+    }
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return efail(nullptr, HD_EHIP, "hipStreamCreate");
+    }
+    std::vector<HdDev> st((size_t)4 * n);
+    for (int s = 0; s < 4; ++s)
+        for (int k = 0; k < n; ++k) {
+            HdDev d = e->m[k]->d;
+            if (s & 1) std::swap(d.t_u, d.tmp_u);
+            if (s & 2) std::swap(d.t_v, d.tmp_v);
+            st[(size_t)s * n + k] = d;
+        }
+    if (hipMalloc((void**)&e->dv, st.size() * sizeof(HdDev)) != hipSuccess ||
+        hipMemcpy(e->dv, st.data(), st.size() * sizeof(HdDev), hipMemcpyHostToDevice) != hipSuccess) {
+        hd_ensemble_destroy(e);
+        return efail(nullptr, HD_ENOMEM, "ensemble device tables");
+    }
+    *out = e;
+    return HD_OK;
+}
+
+int hd_ensemble_destroy(hd_ensemble* e)
+{
+    if (!e) return HD_OK;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (auto& g : e->graphs) hipGraphExecDestroy(g.second);
+    for (auto& r : e->prof.pending) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+    for (auto ev : e->prof.pool) hipEventDestroy(ev);
+    if (e->dv) hipFree(e->dv);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return HD_OK;
+}
+
+const char* hd_ensemble_last_error(const hd_ensemble* e) { return e ? e->err.c_str() : g_err.c_str(); }
+
+int hd_ensemble_run(hd_ensemble* e, double t, int nsteps, const hd_step_opts* o, int counted, double* t_out)
+{
+    if (!e || !o || nsteps < 0) return efail(e, HD_EINVAL, "hd_ensemble_run: bad arguments");
+    if (hipSetDevice(e->device) != hipSuccess) return efail(e, HD_EHIP, "hipSetDevice");
+    hd_ctx* c0 = e->m[0];
+    double dT = c0->P.dT;
+    for (size_t mi = 0; mi < e->m.size(); ++mi) {
+        hd_ctx* c = e->m[mi];
+        for (int k = 0; k < nsteps; ++k) {
+            double tn = t + k * dT;
+            int rc;
+            if ((rc = check_series(c, tn + dT * 0.5, *o)) || (rc = check_series(c, tn + dT, *o)))
+                return efail(e, rc, "member " + std::to_string(mi) + ": " + c->err);
+        }
+        /* work queued on a member's own stream comes first */
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return efail(e, HD_EHIP, "member sync");
+    }
+    int count = counted ? 1 : 0;
+    Exec x{e->stream, e->dv, (int)e->m.size(), e->N, e->M, 0, c0->solver, c0->solver_v1, e->run_blocks_u,
+           e->run_blocks_v, c0->bw, &e->prof, nullptr};
+    set_time(x, t);
+    std::string why;
+    hipError_t err = run_steps(x, e->graphs, t, nsteps, *o, dT, count,
+                               [&](double tt) { return make_key(c0, tt, *o, count); }, why);
+    if (err != hipSuccess) return efail(e, HD_EHIP, why + ": " + hipGetErrorString(err));
+    if ((err = hipGetLastError()) != hipSuccess) return efail(e, HD_EHIP, hipGetErrorString(err));
+    if ((err = hipStreamSynchronize(e->stream)) != hipSuccess) return efail(e, HD_EHIP, hipGetErrorString(err));
+    for (size_t k = 0; k < e->m.size(); ++k) {
+        int rc = check_flags(e->m[k]);
+        if (rc) return efail(e, rc, "member " + std::to_string(k) + ": " + e->m[k]->err);
+    }
+    if (t_out) *t_out = t;
+    return HD_OK;
+}
+
+int hd_ensemble_profile(hd_ensemble* e, int enable)
+{
+    if (!e) return HD_EINVAL;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return efail(e, HD_EHIP, "sync");
+    prof_reset(e->prof, enable != 0);
+    return HD_OK;
+}
+
+int hd_ensemble_kernel_times(hd_ensemble* e, int n, const char** names, double* total_ms, int* launches)
+{
+    if (!e) return HD_EINVAL;
+    if (hipStreamSynchronize(e->stream) != hipSuccess) return efail(e, HD_EHIP, "sync");
+    int k = 0;
+    hipError_t err = prof_collect(e->prof, n, names, total_ms, launches, &k);
+    if (err != hipSuccess) return efail(e, HD_EHIP, hipGetErrorString(err));
     return k;
 }
 
@@ -740,7 +1138,7 @@ int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const d
 {
     if (nsys < 0 || nrows < 0 || !off || !sn || !A || !B || !C || !D || !x) return fail(nullptr, HD_EINVAL, "bad args");
     for (int s = 0; s < nsys; ++s)
-        if (off[s] < 0 || sn[s] >= 0 && off[s] + sn[s] >= nrows) return fail(nullptr, HD_EINVAL, "system out of range");
+        if (off[s] < 0 || (sn[s] >= 0 && off[s] + sn[s] >= nrows)) return fail(nullptr, HD_EINVAL, "system out of range");
     if (hipSetDevice(device) != hipSuccess) return fail(nullptr, HD_EHIP, "hipSetDevice");
     std::vector<void*> bufs;
     auto mk = [&](size_t bytes) -> void* {

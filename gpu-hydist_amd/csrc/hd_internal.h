@@ -43,35 +43,44 @@ struct HdDev {
     int* run_cnt;                   /* [0]: entries of runs_u, [1]: entries of runs_v */
     int* flags;                     /* [0]: segment overflow, [1]: boundary range */
     long long* wet_count;
+    double* tdev;                   /* device time of the current half-step */
+    int *cand_cnt, *cand;           /* wetting candidates per line (64 slots) */
+    unsigned char* dryflag;         /* drying flag per cell */
+};
+
+/* A launch over a batch of contexts (one or an ensemble): dv points to n
+ * device copies of HdDev, kernels take theirs from dv[blockIdx.z]; N, M are
+ * the largest grid sides of the batch and size the launch. */
+struct HdBatch {
+    const HdDev* dv;
+    int n, N, M;
 };
 
 /* kernel launchers (hd_kernels.hip, hd_sediment.hip), one kernel each */
-void hd_launch_onetime_init(const HdDev& d, hipStream_t s);
-void hd_launch_find_limits(const HdDev& d, hipStream_t s); /* segments + fs_edges */
-void hd_launch_segments(const HdDev& d, int count, hipStream_t s);
-void hd_launch_fs_edges(const HdDev& d, hipStream_t s);
-void hd_launch_htuongdoi(const HdDev& d, hipStream_t s);
-void hd_launch_preprocess(const HdDev& d, hipStream_t s);
-void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s);
-void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
-void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
-void hd_launch_adi_run(const HdDev& d, int isU, int sidx, int eidx, int nblocks, hipStream_t s);
-void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
-void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
-                              unsigned char* dryflag, hipStream_t s);
-void hd_launch_explicit(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
-void hd_launch_margins(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s);
-void hd_launch_wetting(const HdDev& d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag,
-                       hipStream_t s);
-void hd_launch_uvz_dry(const HdDev& d, int kd, int kn, const unsigned char* dryflag, hipStream_t s);
-void hd_launch_vth_hesok(const HdDev& d, hipStream_t s);
-void hd_launch_fs_coef(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s);
-void hd_launch_fs_solve(const HdDev& d, int which, hipStream_t s);
-void hd_launch_fs_line(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s);
-void hd_launch_update_fs(const HdDev& d, hipStream_t s);
-void hd_launch_calc_qb(const HdDev& d, int qb_mode, hipStream_t s);
-void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStream_t s);
-void hd_launch_wet_count(const HdDev& d, hipStream_t s);
+void hd_launch_onetime_init(const HdBatch& b, hipStream_t s);
+void hd_launch_find_limits(const HdBatch& b, hipStream_t s); /* run-table reset + segments (+ FS edges) */
+void hd_launch_segments(const HdBatch& b, int count, hipStream_t s); /* also the FS edge interpolations */
+void hd_launch_htuongdoi(const HdBatch& b, hipStream_t s);
+void hd_launch_preprocess(const HdBatch& b, hipStream_t s);
+void hd_launch_boundary_t(const HdBatch& b, int analytic, hipStream_t s);
+void hd_launch_adi_coef(const HdBatch& b, int isU, int shift, hipStream_t s);
+void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s);
+void hd_launch_adi_line(const HdBatch& b, int isU, int shift, hipStream_t s);
+void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
+void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, hipStream_t s);
+void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s);
+void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
+void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
+void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, hipStream_t s);
+void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);
+void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
+void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s);
+void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
+void hd_launch_update_fs(const HdBatch& b, hipStream_t s);
+void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s);
+void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s);
+void hd_launch_wet_count(const HdBatch& b, hipStream_t s);
+void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s);
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
                            const double* C, const double* D, double* x, double* Ap, double* Bp, hipStream_t s);
 

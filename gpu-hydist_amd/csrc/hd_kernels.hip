@@ -32,8 +32,9 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
 
 /* Onetime_init, support_funcs.cu:20-58: every cell ends as khouot = 2 except
  * the cells of [1,N+1]x[1,M+1] whose 4-node mean depth exceeds NANGDAY. */
-__global__ void k_onetime_init(HdDev d)
+__global__ void k_onetime_init(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
@@ -63,8 +64,9 @@ __global__ void k_onetime_init(HdDev d)
 }
 
 /* Htuongdoi, support_funcs.cu:338-354 (init form, from z). */
-__global__ void k_htuongdoi(HdDev d)
+__global__ void k_htuongdoi(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int i = blockIdx.y + 1;
     int W = d.W;
@@ -106,8 +108,9 @@ __device__ void preprocess_edge(const HdDev& d, int edge)
 
 /* preprocess_data, support_funcs.cu:433-542: the four edges (independent hi
  * ranges) in parallel, then hmax_* = 1/(ros * max htaiz_bd) per edge. */
-__global__ void k_preprocess(HdDev d)
+__global__ void k_preprocess(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int e = threadIdx.x;
     if (e < 4) preprocess_edge(d, e);
     if (e != 0) return;
@@ -206,8 +209,10 @@ __device__ void edge_analytic(const HdDev& d, int edge, int i, double t)
  * boundaries), one workgroup; the edges run in the reference's order
  * up, down, left, right, so a corner cell keeps the later edge's value.
  * The half-step time t = t_prev + dT/2 is advanced here, on the device. */
-__global__ void k_boundary(HdDev d, double* tdev, int analytic)
+__global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
 {
+    const HdDev& d = dv[blockIdx.z];
+    double* tdev = d.tdev;
     __shared__ double t_s;
     if (threadIdx.x == 0) t_s = tdev[0] + d.P.dT * 0.5;
     __syncthreads();
@@ -411,8 +416,10 @@ __device__ __forceinline__ void wall_flags(const HdDev& d, bool isU, int line, i
  * Layout: wall/Q start: row 2k = continuity of k, 2k+1 = momentum of k; open
  * start: 2k = momentum of k, 2k+1 = continuity of k+1. */
 template <bool isU>
-__global__ void __launch_bounds__(256) k_adi_coef(HdDev d, int sidx, int eidx)
+__global__ void __launch_bounds__(256) k_adi_coef(const HdDev* __restrict__ dv, int shift)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
     __shared__ int s_cuoi[64][24];
@@ -524,8 +531,10 @@ __global__ void __launch_bounds__(256) k_adi_coef(HdDev d, int sidx, int eidx)
  * forward elimination over each run, back substitution writing t_z and t_u
  * (UZ) or t_v (VZ) as each unknown is resolved. */
 template <bool isU>
-__global__ void __launch_bounds__(64) k_adi_solve(HdDev d, int sidx, int eidx)
+__global__ void __launch_bounds__(64) k_adi_solve(const HdDev* __restrict__ dv, int shift)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > eidx) return;
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
@@ -673,8 +682,10 @@ __device__ __forceinline__ Row4 cell_row(const Pre& a, const Pre& b, double f4, 
  * only the schedule changes.  Elimination factors of rows beyond kCap spill
  * to the global Ap/Bp line buffers. */
 template <bool isU>
-__global__ void __launch_bounds__(64) k_adi_line(HdDev d, int sidx, int eidx)
+__global__ void __launch_bounds__(64) k_adi_line(const HdDev* __restrict__ dv, int shift)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     constexpr int CH = 63;
     constexpr int kCap = 2048;
     __shared__ double sP[7][64];
@@ -838,8 +849,10 @@ __global__ void __launch_bounds__(64) k_adi_line(HdDev d, int sidx, int eidx)
  * depths TZ_l / TZ_r loaded at the start, so no lane reads a value another
  * lane of this launch wrote. */
 template <bool isU>
-__global__ void __launch_bounds__(64) k_adi_run(HdDev d, int sidx, int eidx)
+__global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, int shift)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     constexpr int CH = 127;
     constexpr int RW = 2 * CH;
     __shared__ double sP[7][CH + 1];
@@ -1021,13 +1034,16 @@ __global__ void __launch_bounds__(64) k_adi_run(HdDev d, int sidx, int eidx)
  * (support_funcs.cu:114-127 / 173-185) for every cell, and records the
  * positions where the serial wetting pass can act (a neighbour along the
  * reset direction has khouot == 1). */
-__global__ void __launch_bounds__(256) k_normalize(HdDev d, int isU, int post, int horizontal, int* cand_cnt,
-                                                    int* cand, unsigned char* dryflag)
+__global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv, int isU, int post, int horizontal)
 {
+    const HdDev& d = dv[blockIdx.z];
+    int* cand_cnt = d.cand_cnt;
+    int* cand = d.cand;
+    unsigned char* dryflag = d.dryflag;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
-    if (j >= M + 3) return;
+    if (j >= M + 3 || i >= N + 3) return;
     int p = i * W + j;
     const double* val = isU ? d.t_u : d.t_v;
     double* out = isU ? d.tmp_u : d.tmp_v;
@@ -1150,8 +1166,10 @@ __device__ double u_solver(const HdDev& d, int row, int col)
  * thread per cell, the cell's run from an LDS copy of the line tables; cells
  * first <= k < last of runs with first < last. */
 template <bool isU>
-__global__ void __launch_bounds__(256) k_explicit(HdDev d, int sidx, int eidx)
+__global__ void __launch_bounds__(256) k_explicit(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2, eidx = isU ? d.M : d.N;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
     __shared__ int s_cuoi[64][24];
@@ -1205,8 +1223,10 @@ __global__ void __launch_bounds__(256) k_explicit(HdDev d, int sidx, int eidx)
 /* update_margin_elem_V (UVZSolver_multithread.cu:1008-1052) or
  * update_margin_elem_U (:970-1006): one lane per line, runs in order. */
 template <bool isU>
-__global__ void k_margins(HdDev d, int sidx, int eidx)
+__global__ void k_margins(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = 2, eidx = isU ? d.M : d.N;
     int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > eidx) return;
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
@@ -1248,8 +1268,12 @@ __global__ void k_margins(HdDev d, int sidx, int eidx)
  * and are visited here in increasing order, re-checking the current state, so
  * cascades along the line happen exactly as in the serial loop.  A line with
  * more candidates than fit the list is scanned in full. */
-__global__ void k_wetting(HdDev d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag)
+__global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
 {
+    const HdDev& d = dv[blockIdx.z];
+    int* cand_cnt = d.cand_cnt;
+    int* cand = d.cand;
+    unsigned char* dryflag = d.dryflag;
     int line = blockIdx.x * blockDim.x + threadIdx.x;
     int N = d.N, M = d.M, W = d.W;
     int lo = horizontal ? 1 : 1, hi = horizontal ? N : M;
@@ -1312,12 +1336,14 @@ __global__ void k_wetting(HdDev d, int horizontal, int* cand_cnt, int* cand, uns
  * and t_v at (a,b-1),(a,b); each cell applies the flags of itself and of the
  * neighbours that write it, then copies.  Face depths read the new z (t_z)
  * inside [0,N]x[0,M] and the untouched z outside, as after update_uvz. */
-__global__ void __launch_bounds__(256) k_uvz(HdDev d, int kd, int kn, const unsigned char* dryflag)
+__global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int kd, int kn)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const unsigned char* dryflag = d.dryflag;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
-    if (j >= M + 3) return;
+    if (j >= M + 3 || i >= N + 3) return;
     int p = i * W + j;
     if (p == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; } /* the run tables are rebuilt next */
     bool me = dryflag[p];
@@ -1339,6 +1365,30 @@ __global__ void __launch_bounds__(256) k_uvz(HdDev d, int kd, int kn, const unsi
         double zr = (j + 1 <= M) ? d.t_z[p + 1] : d.z[p + 1];
         d.Htdu[p] = (h[i * W + j - 1] + h[p] + zd + zc) * 0.5;
         d.Htdv[p] = (h[(i - 1) * W + j] + h[p] + zr + zc) * 0.5;
+    }
+}
+
+/* After u, v, z were written from outside the step: t_u = u, t_v = v,
+ * t_z = z everywhere; H_moi = htaiz + z on wet cells of [1,N]x[1,M]
+ * (update_h_moi, support_funcs.cu:60-73); face depths from z (Htuongdoi,
+ * support_funcs.cu:338-354). */
+__global__ void __launch_bounds__(256) k_flow_refresh(const HdDev* __restrict__ dv)
+{
+    const HdDev& d = dv[blockIdx.z];
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    int N = d.N, M = d.M, W = d.W;
+    if (j >= M + 3 || i >= N + 3) return;
+    int p = i * W + j;
+    d.t_u[p] = d.u[p];
+    d.t_v[p] = d.v[p];
+    d.t_z[p] = d.z[p];
+    if (i >= 1 && i <= N && j >= 1 && j <= M) {
+        if (d.khouot[p] == 0) d.H_moi[p] = d.htaiz[p] + d.z[p];
+        const double* h = d.h;
+        const double* z = d.z;
+        d.Htdu[p] = (h[i * W + j - 1] + h[p] + z[p + W] + z[p]) * 0.5;
+        d.Htdv[p] = (h[(i - 1) * W + j] + h[p] + z[p + 1] + z[p]) * 0.5;
     }
 }
 
@@ -1378,8 +1428,9 @@ __device__ void fs_d(const HdDev& d, int location, int offset, int sign, int i)
     }
 }
 
-__global__ void k_reset_runs(HdDev d)
+__global__ void k_reset_runs(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     if (threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
 }
 
@@ -1436,6 +1487,24 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
     return ns;
 }
 
+/* Interpolate_FS_ng rows N and 2, then Interpolate_FS_d column M, by one
+ * workgroup (the two reference kernels' order; the column reads the rows'
+ * corner results). */
+__device__ void fs_edges_block(const HdDev& d)
+{
+    int N = d.N, M = d.M;
+    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
+        fs_ng(d, N, N - 1, -1, j);
+    }
+    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
+        fs_ng(d, 2, 2, 1, j);
+    }
+    __syncthreads();
+    int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
+    for (int i = 1 + threadIdx.x; i <= imax; i += blockDim.x) fs_d(d, M, M - 1, -1, i);
+}
+
+
 /* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335):
  * the per-line tables daui / cuoii / moci (rows i = 2..N) and dauj / cuoij /
  * mocj (columns j = 2..M), plus the packed run tables the solvers read
@@ -1445,8 +1514,13 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
  * rows 4b .. 4b+3; with count_wet these waves also add the row's
  * khouot == 0 cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
  * [nrowb, nrowb + ncolb): columns 2 + 4b' ... */
-__global__ void __launch_bounds__(256) k_find_limits(HdDev d, int nrowb, int ncolb, int count_wet)
+__global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
 {
+    const HdDev& d = dv[blockIdx.z];
+    if (blockIdx.x == (unsigned)(nrowb + ncolb)) { /* the FS edge interpolations (independent of the mask) */
+        fs_edges_block(d);
+        return;
+    }
     __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
     const int N = d.N, M = d.M, W = d.W, SL = d.SL;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -1496,28 +1570,18 @@ __global__ void __launch_bounds__(256) k_find_limits(HdDev d, int nrowb, int nco
     if (lane < n && at < cap) table[at] = make_int4(line, s_f[wv][lane], s_l[wv][lane], 0);
 }
 
-__global__ void __launch_bounds__(256) k_fs_edges(HdDev d)
-{
-    int N = d.N, M = d.M;
-    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
-        fs_ng(d, N, N - 1, -1, j);
-    }
-    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
-        fs_ng(d, 2, 2, 1, j);
-    }
-    __syncthreads();
-    int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
-    for (int i = 1 + threadIdx.x; i <= imax; i += blockDim.x) fs_d(d, M, M - 1, -1, i);
-}
 
-__global__ void k_wet_count(HdDev d)
+__global__ void k_wet_count(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int i = blockIdx.x;
     int lane = threadIdx.x;
+    if (i >= d.N + 3) return;
     int c = 0;
     for (int j = lane; j < d.M + 3; j += kWave) c += (d.khouot[i * d.W + j] == 0);
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
-    if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
+    /* slot 1 of the counter pair: the one-shot count */
+    if (lane == 0 && c) atomicAdd((unsigned long long*)(d.wet_count + 1), (unsigned long long)c);
 }
 
 /* ================================================================ batched Thomas */
@@ -1548,114 +1612,112 @@ __global__ void k_thomas_batch(int nsys, const int* off, const int* sn, const do
 
 /* ================================================================ launchers */
 
-static inline dim3 grid2d(const HdDev& d, int bx) { return dim3((d.M + 3 + bx - 1) / bx, d.N + 3); }
+/* Every launch covers a batch of contexts: grid.z indexes the batch, the x/y
+ * extents are sized for the largest grid of the batch and each kernel
+ * returns early outside its own context's extents. */
+static inline dim3 grid2d(const HdBatch& b, int bx) { return dim3((b.M + 3 + bx - 1) / bx, b.N + 3, b.n); }
 
-void hd_launch_onetime_init(const HdDev& d, hipStream_t s)
+void hd_launch_onetime_init(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_onetime_init, grid2d(d, 128), dim3(128), 0, s, d);
+    hipLaunchKernelGGL(k_onetime_init, grid2d(b, 128), dim3(128), 0, s, b.dv);
 }
 
-void hd_launch_htuongdoi(const HdDev& d, hipStream_t s)
+void hd_launch_htuongdoi(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_htuongdoi, dim3((d.M + 127) / 128, d.N), dim3(128), 0, s, d);
+    hipLaunchKernelGGL(k_htuongdoi, dim3((b.M + 127) / 128, b.N, b.n), dim3(128), 0, s, b.dv);
 }
 
-void hd_launch_preprocess(const HdDev& d, hipStream_t s)
+void hd_launch_preprocess(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_preprocess, dim3(1), dim3(64), 0, s, d);
+    hipLaunchKernelGGL(k_preprocess, dim3(1, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
-void hd_launch_find_limits(const HdDev& d, hipStream_t s)
+void hd_launch_find_limits(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reset_runs, dim3(1), dim3(64), 0, s, d);
-    hd_launch_segments(d, 0, s);
-    hd_launch_fs_edges(d, s);
+    hipLaunchKernelGGL(k_reset_runs, dim3(1, 1, b.n), dim3(64), 0, s, b.dv);
+    hd_launch_segments(b, 0, s);
 }
 
-void hd_launch_segments(const HdDev& d, int count, hipStream_t s)
+void hd_launch_segments(const HdBatch& b, int count, hipStream_t s)
 {
-    int nrowb = (d.N + 3 + 3) / 4, ncolb = (d.M - 1 + 3) / 4;
-    hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb), dim3(256), 0, s, d, nrowb, ncolb, count);
+    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + 3) / 4;
+    /* + one block for the FS edge interpolations */
+    hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb + 1, 1, b.n), dim3(256), 0, s, b.dv, nrowb, ncolb, count);
 }
 
-void hd_launch_fs_edges(const HdDev& d, hipStream_t s)
+
+void hd_launch_boundary_t(const HdBatch& b, int analytic, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fs_edges, dim3(1), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_boundary, dim3(1, 1, b.n), dim3(1024), 0, s, b.dv, analytic);
 }
 
-void hd_launch_boundary_t(const HdDev& d, double* tdev, int analytic, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, s, d, tdev, analytic);
-}
-
-void hd_launch_adi_coef(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+void hd_launch_adi_coef(const HdBatch& b, int isU, int shift, hipStream_t s)
 {
     dim3 blk(64, 4);
-    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
-    if (eidx < sidx) return;
-    if (isU) hipLaunchKernelGGL(k_adi_coef<true>, grd, blk, 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_adi_coef<false>, grd, blk, 0, s, d, sidx, eidx);
+    dim3 grd((b.M + 3 + 63) / 64, (b.N + 3 + 15) / 16, b.n);
+    if (isU) hipLaunchKernelGGL(k_adi_coef<true>, grd, blk, 0, s, b.dv, shift);
+    else hipLaunchKernelGGL(k_adi_coef<false>, grd, blk, 0, s, b.dv, shift);
 }
 
-void hd_launch_adi_line(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+void hd_launch_adi_line(const HdBatch& b, int isU, int shift, hipStream_t s)
 {
-    int nl = eidx - sidx + 1;
-    if (nl <= 0) return;
-    if (isU) hipLaunchKernelGGL(k_adi_line<true>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_adi_line<false>, dim3(nl), dim3(64), 0, s, d, sidx, eidx);
+    int nl = (isU ? b.M : b.N) - 1;
+    if (isU) hipLaunchKernelGGL(k_adi_line<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, shift);
+    else hipLaunchKernelGGL(k_adi_line<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
-void hd_launch_adi_run(const HdDev& d, int isU, int sidx, int eidx, int nblocks, hipStream_t s)
+void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s)
 {
-    if (eidx < sidx) return;
-    if (isU) hipLaunchKernelGGL(k_adi_run<true>, dim3(nblocks), dim3(64), 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_adi_run<false>, dim3(nblocks), dim3(64), 0, s, d, sidx, eidx);
+    if (isU) hipLaunchKernelGGL(k_adi_run<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
+    else hipLaunchKernelGGL(k_adi_run<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
-void hd_launch_adi_solve(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s)
 {
-    int nl = eidx - sidx + 1;
-    if (nl <= 0) return;
-    if (isU) hipLaunchKernelGGL(k_adi_solve<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_adi_solve<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+    int nl = (isU ? b.M : b.N) - 1;
+    if (isU) hipLaunchKernelGGL(k_adi_solve<true>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, shift);
+    else hipLaunchKernelGGL(k_adi_solve<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
-void hd_launch_normalize_post(const HdDev& d, int isU, int post, int horizontal, int* cand_cnt, int* cand,
-                              unsigned char* dryflag, hipStream_t s)
+void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_normalize, grid2d(d, 256), dim3(256), 0, s, d, isU, post, horizontal, cand_cnt, cand, dryflag);
+    hipLaunchKernelGGL(k_normalize, grid2d(b, bw), dim3(bw), 0, s, b.dv, isU, post, horizontal);
 }
 
-void hd_launch_explicit(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s)
 {
     dim3 blk(64, 4);
-    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
-    if (isU) hipLaunchKernelGGL(k_explicit<true>, grd, blk, 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_explicit<false>, grd, blk, 0, s, d, sidx, eidx);
+    dim3 grd((b.M + 3 + 63) / 64, (b.N + 3 + 15) / 16, b.n);
+    if (isU) hipLaunchKernelGGL(k_explicit<true>, grd, blk, 0, s, b.dv);
+    else hipLaunchKernelGGL(k_explicit<false>, grd, blk, 0, s, b.dv);
 }
 
-void hd_launch_margins(const HdDev& d, int isU, int sidx, int eidx, hipStream_t s)
+void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s)
 {
-    int nl = eidx - sidx + 1;
-    if (nl <= 0) return;
-    if (isU) hipLaunchKernelGGL(k_margins<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
-    else hipLaunchKernelGGL(k_margins<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d, sidx, eidx);
+    int nl = (isU ? b.M : b.N) - 1;
+    if (isU) hipLaunchKernelGGL(k_margins<true>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
+    else hipLaunchKernelGGL(k_margins<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
-void hd_launch_wetting(const HdDev& d, int horizontal, int* cand_cnt, int* cand, unsigned char* dryflag, hipStream_t s)
+void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)
 {
-    int n = (horizontal ? d.N : d.M) + 1;
-    hipLaunchKernelGGL(k_wetting, dim3((n + 63) / 64), dim3(64), 0, s, d, horizontal, cand_cnt, cand, dryflag);
+    int n = (horizontal ? b.N : b.M) + 1;
+    hipLaunchKernelGGL(k_wetting, dim3((n + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, horizontal);
 }
 
-void hd_launch_uvz_dry(const HdDev& d, int kd, int kn, const unsigned char* dryflag, hipStream_t s)
+void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_uvz, grid2d(d, 256), dim3(256), 0, s, d, kd, kn, dryflag);
+    hipLaunchKernelGGL(k_uvz, grid2d(b, bw), dim3(bw), 0, s, b.dv, kd, kn);
 }
 
-void hd_launch_wet_count(const HdDev& d, hipStream_t s)
+void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_wet_count, dim3(d.N + 3), dim3(64), 0, s, d);
+    hipLaunchKernelGGL(k_flow_refresh, grid2d(b, 256), dim3(256), 0, s, b.dv);
+}
+
+void hd_launch_wet_count(const HdBatch& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wet_count, dim3(b.N + 3, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,

@@ -39,8 +39,9 @@ __device__ double source_nchs(const hd_params& P, double wsm, double Zf, double 
 }
 
 /* Find_VTH (Sediment_Transport.cu:541-568) and hesoK (:508-539). */
-__global__ void __launch_bounds__(256) k_vth_hesok(HdDev d)
+__global__ void __launch_bounds__(256) k_vth_hesok(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
@@ -162,8 +163,10 @@ __device__ FsRow fs_row(const HdDev& d, double t, double s_start, int ketdinh, i
  * _FSi_calculate__mactrix_coeff (:62-165, row lines).  Row of cell k sits at
  * line*tw + k. */
 template <bool isJ>
-__global__ void __launch_bounds__(256) k_fs_coef(HdDev d, const double* tdev, double s_start, int ketdinh)
+__global__ void __launch_bounds__(256) k_fs_coef(const HdDev* __restrict__ dv, double s_start, int ketdinh)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const double* tdev = d.tdev;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
     __shared__ int s_cuoi[64][24];
@@ -226,8 +229,9 @@ __global__ void __launch_bounds__(256) k_fs_coef(HdDev d, const double* tdev, do
  * 305-338 / 168-202), one lane per line.  Runs with last - first < 2 have no
  * rows and are skipped. */
 template <bool isJ>
-__global__ void __launch_bounds__(64) k_fs_solve(HdDev d)
+__global__ void __launch_bounds__(64) k_fs_solve(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
     int line = 3 + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > (isJ ? M - 1 : N - 1)) return;
@@ -287,8 +291,10 @@ __global__ void __launch_bounds__(64) k_fs_solve(HdDev d)
  * substitution and the extraction into tFS with the end copies
  * (Sediment_Transport.cu:305-338 / 168-202), lanes in parallel. */
 template <bool isJ>
-__global__ void __launch_bounds__(64) k_fs_line(HdDev d, const double* tdev, double s_start, int ketdinh)
+__global__ void __launch_bounds__(64) k_fs_line(const HdDev* __restrict__ dv, double s_start, int ketdinh)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const double* tdev = d.tdev;
     constexpr int CH = 64;
     constexpr int kCap = 2048;
     __shared__ double sA[CH], sB[CH], sC[CH], sD[CH];
@@ -376,8 +382,9 @@ __global__ void __launch_bounds__(64) k_fs_line(HdDev d, const double* tdev, dou
 }
 
 /* Update_FS, Sediment_Transport.cu:620-629. */
-__global__ void __launch_bounds__(256) k_update_fs(HdDev d)
+__global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv)
 {
+    const HdDev& d = dv[blockIdx.z];
     int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int i = blockIdx.y + 1;
     if (i > d.N || j > d.M) return;
@@ -386,8 +393,9 @@ __global__ void __launch_bounds__(256) k_update_fs(HdDev d)
 
 /* Calculate_Qb, Sediment_Transport.cu:341-414 (qb_mode REFERENCE keeps the
  * `!VTH` test as written, FIXED uses VTH != 0; corner copies braced). */
-__global__ void __launch_bounds__(256) k_calc_qb(HdDev d, int qb_mode)
+__global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, int qb_mode)
 {
+    const HdDev& d = dv[blockIdx.z];
     int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
     int i = blockIdx.y + 2;
     int N = d.N, M = d.M, W = d.W;
@@ -429,8 +437,10 @@ __global__ void __launch_bounds__(256) k_calc_qb(HdDev d, int qb_mode)
 /* _bed_load via BedLoad, Sediment_Transport.cu:416-506 / 630-639: rows
  * [sidx, eidx-1], cells of row runs with last - first >= 2 (t_u[pos > 0]
  * reads t_u[1], as written). */
-__global__ void __launch_bounds__(256) k_bedload(HdDev d, int ketdinh, int sidx, int eidx)
+__global__ void __launch_bounds__(256) k_bedload(const HdDev* __restrict__ dv, int ketdinh, int slo, int eoff)
 {
+    const HdDev& d = dv[blockIdx.z];
+    const int sidx = slo, eidx = d.N - eoff;
     __shared__ int s_moc;
     __shared__ int s_dau[24], s_cuoi[24];
     int i = sidx + blockIdx.y;
@@ -492,48 +502,49 @@ __global__ void __launch_bounds__(256) k_bedload(HdDev d, int ketdinh, int sidx,
 
 } // namespace
 
-void hd_launch_vth_hesok(const HdDev& d, hipStream_t s)
+void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s)
 {
-    dim3 g2((d.M + 1 + 255) / 256, d.N + 1);
-    hipLaunchKernelGGL(k_vth_hesok, g2, dim3(256), 0, s, d);
+    dim3 g2((b.M + 1 + 255) / 256, b.N + 1, b.n);
+    hipLaunchKernelGGL(k_vth_hesok, g2, dim3(256), 0, s, b.dv);
 }
 
-void hd_launch_fs_coef(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s)
+void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s)
 {
     dim3 blk(64, 4);
-    dim3 grd((d.M + 3 + 63) / 64, (d.N + 3 + 15) / 16);
-    if (which == 0) hipLaunchKernelGGL(k_fs_coef<true>, grd, blk, 0, s, d, tdev, s_start, ketdinh);
-    else hipLaunchKernelGGL(k_fs_coef<false>, grd, blk, 0, s, d, tdev, s_start, ketdinh);
+    dim3 grd((b.M + 3 + 63) / 64, (b.N + 3 + 15) / 16, b.n);
+    if (which == 0) hipLaunchKernelGGL(k_fs_coef<true>, grd, blk, 0, s, b.dv, s_start, ketdinh);
+    else hipLaunchKernelGGL(k_fs_coef<false>, grd, blk, 0, s, b.dv, s_start, ketdinh);
 }
 
-void hd_launch_fs_solve(const HdDev& d, int which, hipStream_t s)
+void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s)
 {
-    int nl = (which == 0 ? d.M : d.N) - 3;
+    int nl = (which == 0 ? b.M : b.N) - 3;
     if (nl <= 0) return;
-    if (which == 0) hipLaunchKernelGGL(k_fs_solve<true>, dim3((nl + 63) / 64), dim3(64), 0, s, d);
-    else hipLaunchKernelGGL(k_fs_solve<false>, dim3((nl + 63) / 64), dim3(64), 0, s, d);
+    if (which == 0) hipLaunchKernelGGL(k_fs_solve<true>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
+    else hipLaunchKernelGGL(k_fs_solve<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
-void hd_launch_update_fs(const HdDev& d, hipStream_t s)
+void hd_launch_update_fs(const HdBatch& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_fs, dim3((d.M + 255) / 256, d.N), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_update_fs, dim3((b.M + 255) / 256, b.N, b.n), dim3(256), 0, s, b.dv);
 }
 
-void hd_launch_calc_qb(const HdDev& d, int qb_mode, hipStream_t s)
+void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_calc_qb, dim3((d.M - 1 + 255) / 256, d.N - 1), dim3(256), 0, s, d, qb_mode);
+    hipLaunchKernelGGL(k_calc_qb, dim3((b.M - 1 + 255) / 256, b.N - 1, b.n), dim3(256), 0, s, b.dv, qb_mode);
 }
 
-void hd_launch_bedload(const HdDev& d, int ketdinh, int sidx, int eidx, hipStream_t s)
+void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s)
 {
-    int nr = eidx - sidx;
-    if (nr > 0) hipLaunchKernelGGL(k_bedload, dim3((d.M - 1 + 255) / 256, nr), dim3(256), 0, s, d, ketdinh, sidx, eidx);
+    int nr = b.N - eoff - slo;
+    if (nr > 0)
+        hipLaunchKernelGGL(k_bedload, dim3((b.M - 1 + 255) / 256, nr, b.n), dim3(256), 0, s, b.dv, ketdinh, slo, eoff);
 }
 
-void hd_launch_fs_line(const HdDev& d, int which, const double* tdev, double s_start, int ketdinh, hipStream_t s)
+void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s)
 {
-    int nl = (which == 0 ? d.M : d.N) - 3;
+    int nl = (which == 0 ? b.M : b.N) - 3;
     if (nl <= 0) return;
-    if (which == 0) hipLaunchKernelGGL(k_fs_line<true>, dim3(nl), dim3(64), 0, s, d, tdev, s_start, ketdinh);
-    else hipLaunchKernelGGL(k_fs_line<false>, dim3(nl), dim3(64), 0, s, d, tdev, s_start, ketdinh);
+    if (which == 0) hipLaunchKernelGGL(k_fs_line<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
+    else hipLaunchKernelGGL(k_fs_line<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
 }

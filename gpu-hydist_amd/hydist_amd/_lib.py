@@ -22,6 +22,9 @@ EXPORTS = (
     "hd_field_info", "hd_upload", "hd_download", "hd_get_hmax", "hd_init", "hd_prepare", "hd_step", "hd_run",
     "hd_halfstep", "hd_bed_change", "hd_run_counted", "hd_reset_wet_counter", "hd_wet_cell_steps",
     "hd_wet_count", "hd_sync", "hd_struct_ptrs", "hd_profile", "hd_kernel_times", "hd_thomas_batch",
+    "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
+    "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
+    "hd_clone", "hd_autotune", "hd_set_launch",
 )
 
 _lib = None
@@ -70,6 +73,17 @@ def load() -> ctypes.CDLL:
         "hd_profile": (ci, [vp, ci]),
         "hd_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
         "hd_thomas_batch": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
+        "hd_flow_refresh": (ci, [vp]),
+        "hd_clone": (ci, [vp, ctypes.POINTER(vp)]),
+        "hd_autotune": (ci, [vp, cd, po, ci, ip, ip, dp]),
+        "hd_set_launch": (ci, [vp, ci, ci]),
+        "hd_sediment_run": (ci, [vp, cd, ci, po, dp]),
+        "hd_ensemble_create": (ci, [ctypes.POINTER(vp), ci, ctypes.POINTER(vp)]),
+        "hd_ensemble_destroy": (ci, [vp]),
+        "hd_ensemble_last_error": (cp, [vp]),
+        "hd_ensemble_run": (ci, [vp, cd, ci, po, ci, dp]),
+        "hd_ensemble_profile": (ci, [vp, ci]),
+        "hd_ensemble_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -179,6 +193,33 @@ class Solver:
     def sync(self):
         self._chk(load().hd_sync(self.h))
 
+    def clone(self):
+        """A second Solver holding a copy of this one's device state."""
+        h = ctypes.c_void_p()
+        self._chk(load().hd_clone(self.h, ctypes.byref(h)))
+        other = Solver.__new__(Solver)
+        other.N, other.M, other._params, other.case = self.N, self.M, self._params, self.case
+        other.h = h
+        return other
+
+    def autotune(self, t, opts, steps=10):
+        """Pick the launch geometry on a clone; returns (rb_mult, bw, ms/step)."""
+        rb, bw, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._chk(load().hd_autotune(self.h, t, ctypes.byref(opts), steps, ctypes.byref(rb), ctypes.byref(bw),
+                                     ctypes.byref(ms)))
+        return rb.value, bw.value, ms.value
+
+    def set_launch(self, rb_mult, bw):
+        self._chk(load().hd_set_launch(self.h, rb_mult, bw))
+
+    def flow_refresh(self):
+        self._chk(load().hd_flow_refresh(self.h))
+
+    def sediment_run(self, t, nsteps, opts):
+        tout = ctypes.c_double()
+        self._chk(load().hd_sediment_run(self.h, t, nsteps, ctypes.byref(opts), ctypes.byref(tout)))
+        return tout.value
+
     def wet_count(self):
         n = ctypes.c_longlong()
         self._chk(load().hd_wet_count(self.h, ctypes.byref(n)))
@@ -227,3 +268,59 @@ def thomas_batch(systems, device=0):
     check(L.hd_thomas_batch(device, len(systems), off_a.ctypes.data_as(ip), sn_a.ctypes.data_as(ip),
                             *(c.ctypes.data_as(dp) for c in cat), x.ctypes.data_as(dp), rows))
     return [x[o:o + n + 1] for o, n in zip(off, sn)]
+
+
+class Ensemble:
+    """Independent cases stepped together on one device: every kernel of a
+    step is one launch over all members (hd_ensemble_*).  Members are
+    Solver objects, already initialised; read them back with Solver.get."""
+
+    def __init__(self, members):
+        L = load()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        self.h = ctypes.c_void_p()
+        rc = L.hd_ensemble_create(arr, len(self.members), ctypes.byref(self.h))
+        if rc != 0:
+            raise HydistError(rc, L.hd_ensemble_last_error(None).decode())
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise HydistError(rc, load().hd_ensemble_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            load().hd_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, t, nsteps, opts, counted=False):
+        tout = ctypes.c_double()
+        self._chk(load().hd_ensemble_run(self.h, t, nsteps, ctypes.byref(opts), 1 if counted else 0,
+                                         ctypes.byref(tout)))
+        return tout.value
+
+    def wet_cell_steps(self):
+        return sum(m.wet_cell_steps() for m in self.members)
+
+    def reset_wet_counter(self):
+        for m in self.members:
+            m.reset_wet_counter()
+
+    def profile(self, on=True):
+        self._chk(load().hd_ensemble_profile(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        n = 64
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        k = load().hd_ensemble_kernel_times(self.h, n, names, ms, cnt)
+        if k < 0:
+            self._chk(k)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}

@@ -12,7 +12,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 OUT = os.path.join(HERE, "_build", "libhydist.so")
-SOURCES = ("hd_api.hip", "hd_kernels.hip", "hd_sediment.hip")
+SOURCES = ("hd_api.hip", "hd_kernels.hip", "hd_sediment.hip", "hd_cnn.hip")
 ARCH = os.environ.get("HYDIST_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -1,0 +1,280 @@
+"""Surrogate-accelerated morphodynamics: a residual CNN (hydist_amd.cnn, BF16
+MFMA kernels) stands in for the FP64 ADI hydrodynamics during a bed-change
+phase, and the plain ADI step judges it.
+
+The plain path is the drop-in (Solver.run: hydrodynamics + sediment + bed
+change every step).  The accelerated path, per macro step of K steps:
+
+1. the network predicts u, v, z at t + K dT from the state at t and the
+   boundary forcing at t + K dT (hd_cnn_predict_into_ctx writes them into
+   the context and refreshes the derived depths);
+2. the sediment half of K steps (FS transport, bed change) runs on that
+   frozen flow (Solver.sediment_run);
+3. every `check_every` network calls one plain FP64 ADI step runs from the
+   surrogate state; its change of the water level over the wet cells (the
+   state's departure from the FP64 dynamics) and the relative change of the
+   wet volume (mass balance) are compared with the bounds; past either bound
+   the driver reverts to plain ADI for `fallback_macro` macro steps.
+
+Training data are (state, state + K steps) pairs from plain ADI runs of the
+river with seeded inflow variants, kept on the device (hd_cnn_data_alloc /
+hd_cnn_capture); training uses the library's own forward, backward and Adam
+kernels.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+from typing import List, Optional
+
+import numpy as np
+
+from . import cnn, synth
+from ._lib import Solver
+from .case import BED_EVERY_STEP
+
+
+def series_at(series: np.ndarray, t: float) -> float:
+    """Hourly series value at time t, linear in the hour (the boundary
+    kernel's interpolation, support_funcs.cu:547-565)."""
+    t1 = int(t / 3600.0)
+    t2 = (t - 3600.0 * t1) / 3600.0
+    return float(series[t1] * (1.0 - t2) + series[t1 + 1] * t2)
+
+
+def forcing(case, t: float):
+    """(discharge, water level) boundary forcing of a river case at t."""
+    a = case.arrays
+    if "bc_left" in a and np.any(a["bc_left"]):
+        return series_at(a["bc_left"], t), series_at(a["bc_right"], t)
+    return series_at(a["bc_down"], t), series_at(a["bc_up"], t)
+
+
+SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=0.01, s_q=3000.0, s_h=10.0)
+
+
+def river_variant(k: int, scale: float = 1.0, hours: float = 30.0, seed: int = 50):
+    """Config-3 river (sediment, bed change every step from hour 2) with a
+    seeded inflow hydrograph: mean, amplitude and period drawn per variant."""
+    rng = np.random.default_rng(seed + k)
+    qm = 3000.0 * scale * float(rng.uniform(0.75, 1.25))
+    qa = qm * float(rng.uniform(0.15, 0.4))
+    c = synth.config(3, scale=scale, hours=hours, q_mean=qm, q_amp=qa)
+    c.bed_mode = BED_EVERY_STEP
+    c.name = f"{c.name}_v{k}"
+    return c
+
+
+@dataclasses.dataclass
+class TrainReport:
+    samples: int
+    epochs: int
+    train_loss: List[float]
+    val_loss: float
+    val_loss_persistence: float
+    gen_s: float
+    train_s: float
+
+
+def build_dataset(net: cnn.Network, variants: int, pairs: int, K: int, warm_h: float = 2.0,
+                  scale: float = 1.0, log=print):
+    """Fill the network's sample store with `variants * pairs` (state,
+    state + K steps) pairs from plain FP64 ADI runs; returns the number of
+    samples."""
+    n = variants * pairs
+    net.data_alloc(n)
+    idx = 0
+    for v in range(variants):
+        c = river_variant(v, scale=scale)
+        g = Solver(c)
+        g.init()
+        op = c.opts()
+        t = g.run(0.0, int(round(warm_h * 3600.0 / c.dT)), op)
+        for _ in range(pairs):
+            q, zb = forcing(c, t + K * c.dT)
+            net.capture(idx, 0, g, q, zb)
+            t = g.run(t, K, op)
+            net.capture(idx, 1, g)
+            idx += 1
+        g.close()
+        log(f"  variant {v}: {pairs} pairs up to t = {t / 3600.0:.2f} h")
+    return n
+
+
+def train(net: cnn.Network, n: int, epochs: int, batch: int, lr: float, val_frac: float = 0.1,
+          seed: int = 0, log=print):
+    """Adam on shuffled mini-batches of the sample store; the last val_frac
+    of every variant block is held out (here: the last samples)."""
+    rng = np.random.default_rng(seed)
+    nval = max(1, int(n * val_frac))
+    tr = np.arange(n - nval)
+    va = np.arange(n - nval, n)
+    hist = []
+    for ep in range(epochs):
+        rng.shuffle(tr)
+        tot, cnt = 0.0, 0
+        lr_e = lr * (0.5 * (1 + math.cos(math.pi * ep / max(1, epochs))))
+        for s in range(0, len(tr) - batch + 1, batch):
+            tot += net.train_samples([int(i) for i in tr[s:s + batch]], lr_e)
+            cnt += 1
+        hist.append(tot / max(1, cnt))
+        log(f"  epoch {ep}: train loss {hist[-1]:.3e}")
+    val = float(np.mean([net.eval_samples([int(i)]) for i in va]))
+    return hist, val, va
+
+
+def persistence_loss(net: cnn.Network, idx) -> float:
+    """Loss of the 'no change' prediction (network output forced to zero)
+    on the given samples: the method without the network."""
+    saved = [net.get_layer(l) for l in range(net.layers)]
+    w, b = saved[-1]
+    net.set_layer(net.layers - 1, np.zeros_like(w), np.zeros_like(b))
+    val = float(np.mean([net.eval_samples([int(i)]) for i in idx]))
+    net.set_layer(net.layers - 1, w, b)
+    return val
+
+
+@dataclasses.dataclass
+class RunReport:
+    sim_hours: float
+    wall_s: float
+    sim_hours_per_wall_s: float
+    macro_steps: int = 0
+    surrogate_calls: int = 0
+    checks: int = 0
+    fallbacks: int = 0
+    adi_steps: int = 0
+
+
+def plain_run(case, t0: float, t1: float, g: Solver) -> RunReport:
+    op = case.opts()
+    n = int(round((t1 - t0) / case.dT))
+    g.sync()
+    w0 = time.perf_counter()
+    g.run(t0, n, op)
+    g.sync()
+    w = time.perf_counter() - w0
+    return RunReport((t1 - t0) / 3600.0, w, (t1 - t0) / 3600.0 / w, adi_steps=n)
+
+
+def wet_volume(g: Solver) -> float:
+    z, kh, hz = g.get("z"), g.get("khouot"), g.get("htaiz")
+    m = kh == 0
+    return float((hz[m] + z[m]).sum())
+
+
+def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: int, check_every: int = 10,
+                  z_bound: float = 5e-3, mass_bound: float = 1e-4, fallback_macro: int = 5) -> RunReport:
+    """Bed-change phase from t0 to t1 with the network standing in for the
+    hydrodynamics (module docstring)."""
+    op = case.opts()
+    rep = RunReport(0, 0, 0)
+    t = t0
+    calls_since_check = 0
+    adi_left = 0
+    g.sync()
+    w0 = time.perf_counter()
+    while t < t1 - 1e-9:
+        k = min(K, int(round((t1 - t) / case.dT)))
+        rep.macro_steps += 1
+        if adi_left > 0 or k < K:
+            t = g.run(t, k, op)
+            rep.adi_steps += k
+            adi_left = max(0, adi_left - 1)
+            continue
+        q, zb = forcing(case, t + K * case.dT)
+        net.predict_into(g, q, zb)
+        rep.surrogate_calls += 1
+        calls_since_check += 1
+        t = g.sediment_run(t, K, op)
+        if calls_since_check >= check_every:
+            calls_since_check = 0
+            rep.checks += 1
+            z0 = g.get("z")
+            v0 = wet_volume(g)
+            kh = g.get("khouot")
+            t = g.run(t, 1, op)
+            rep.adi_steps += 1
+            z1 = g.get("z")
+            m = (kh == 0) & (g.get("khouot") == 0)
+            dz = float(np.abs(z1[m] - z0[m]).max()) if m.any() else 0.0
+            dv = abs(wet_volume(g) - v0) / max(v0, 1e-30)
+            if dz > z_bound or dv > mass_bound:
+                rep.fallbacks += 1
+                adi_left = fallback_macro
+    g.sync()
+    rep.wall_s = time.perf_counter() - w0
+    rep.sim_hours = (t - t0) / 3600.0
+    rep.sim_hours_per_wall_s = rep.sim_hours / rep.wall_s
+    return rep
+
+
+def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 3, batch: int = 4,
+             lr: float = 1e-3, days: float = 7.0, layers: int = 8, log=print) -> dict:
+    """Train the surrogate on plain-ADI pairs and compare the accelerated
+    bed-change run with the plain one on the config-3 river (bed change every
+    step, from hour 2): simulated hours per wall second of both, and the bed
+    (dH) difference after `days` days."""
+    warm = 2.0
+    case = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0)
+    case.bed_mode = BED_EVERY_STEP
+    H, W = case.N + 3, case.M + 3
+    net = cnn.Network(H, W, batch=batch, layers=layers, seed=1)
+    net.set_scales(**SCALES)
+    t0 = time.perf_counter()
+    n = build_dataset(net, variants, pairs, K, scale=scale, log=log)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist, val, va = train(net, n, epochs, batch, lr, log=log)
+    train_s = time.perf_counter() - t0
+    pers = persistence_loss(net, va)
+    log(f"  validation loss {val:.3e} (persistence {pers:.3e})")
+    tb = warm * 3600.0
+    te = tb + days * 86400.0
+    op = case.opts()
+    g1 = Solver(case)
+    g1.init()
+    g1.run(0.0, int(round(tb / case.dT)), op)
+    plain = plain_run(case, tb, te, g1)
+    log(f"  plain ADI: {plain.sim_hours:.1f} h in {plain.wall_s:.1f} s")
+    g2 = Solver(case)
+    g2.init()
+    g2.run(0.0, int(round(tb / case.dT)), op)
+    sur = surrogate_run(case, tb, te, g2, net, K)
+    log(f"  surrogate: {sur.sim_hours:.1f} h in {sur.wall_s:.1f} s, {sur.surrogate_calls} calls, "
+        f"{sur.fallbacks} fallbacks")
+    d1, d2, kh = g1.get("dH"), g2.get("dH"), g1.get("khouot")
+    wet = kh == 0
+    bed_err = float(np.abs(d1[wet] - d2[wet]).max())
+    bed_change = float(np.abs(d1[wet]).max())
+    net.close()
+    return {
+        "workload": f"{case.name} bed change {days:g} d (every step), K={K}",
+        "plain_sim_hours_per_wall_s": plain.sim_hours_per_wall_s,
+        "surrogate_sim_hours_per_wall_s": sur.sim_hours_per_wall_s,
+        "speedup": sur.sim_hours_per_wall_s / plain.sim_hours_per_wall_s,
+        "bed_error_max_m": bed_err, "bed_change_max_m": bed_change,
+        "surrogate_calls": sur.surrogate_calls, "checks": sur.checks, "fallbacks": sur.fallbacks,
+        "adi_steps_in_surrogate_run": sur.adi_steps,
+        "train": {"samples": n, "epochs": epochs, "loss": hist, "val_loss": val,
+                  "val_loss_persistence": pers, "gen_s": gen_s, "train_s": train_s},
+    }
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--variants", type=int, default=8)
+    ap.add_argument("--pairs", type=int, default=250)
+    ap.add_argument("--K", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--days", type=float, default=7.0)
+    a = ap.parse_args()
+    r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
+                 days=a.days, log=lambda s: print(s, flush=True))
+    print(json.dumps(r))

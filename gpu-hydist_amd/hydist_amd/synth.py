@@ -190,13 +190,16 @@ def channel_case(N: int = 300, M: int = 60, dx: float = 10.0, dt: float = 2.0, n
 
 def river_nodes(N: int, M: int, amp: float, wavelength: float, wmin: float, wmax: float, depth: float,
                 noise: float, n_islands: int, rmin: float, rmax: float, seed: int,
-                edge_margin: int = 20) -> np.ndarray:
+                edge_margin: int = 20, open_floor: float = 0.0) -> np.ndarray:
     """Node depth map (N x M) of a meandering channel running along j: the
     centreline i_c(j) = N/2 + amp sin(2 pi j / wavelength), local width
     varying between wmin and wmax (seeded-phase sinusoid), parabolic cross
     section of maximum depth `depth` measured across the channel axis, seeded
     Gaussian bed noise, and seeded circular dry islands kept at least
-    `edge_margin` cells from the open edges j = 0 and j = M-1."""
+    `edge_margin` cells from the open edges j = 0 and j = M-1.  With
+    open_floor > 0 the straight reaches next to the open edges get that
+    minimum depth across the whole channel width (steep banks), so a large
+    tide does not dry and re-wet cells on the open boundary itself."""
     rng = np.random.default_rng(seed)
     j = np.arange(M, dtype=F64)[None, :]
     i = np.arange(N, dtype=F64)[:, None]
@@ -220,6 +223,9 @@ def river_nodes(N: int, M: int, amp: float, wavelength: float, wmin: float, wmax
     nz[:, :3] = 0.0
     nz[:, -3:] = 0.0
     hn = np.where(inside, np.maximum(hn + nz, 0.0), 0.0)
+    if open_floor > 0.0:
+        reach = (dist < taper) & inside
+        hn = np.where(reach, np.maximum(hn, open_floor), hn)
     jj = np.arange(M)[None, :]
     ii = np.arange(N)[:, None]
     for _ in range(n_islands):
@@ -239,17 +245,18 @@ def river_case(N: int = 980, M: int = 1048, dx: float = 20.0, dt: float = 5.0, h
                rmax: float = 10.0, n_manning: float = 0.03, seed: int = 2, SL: int = 20,
                q_mean: float = 3000.0, q_amp: float = 1000.0, q_period_h: float = 6.0,
                tide_amp: float = 0.5, tide_period_h: float = 12.42, H_TINH: float = 0.05,
-               sediment: bool = False, name: str = "river", axis: str = "i", **header_over) -> Case:
+               sediment: bool = False, name: str = "river", axis: str = "i", open_floor: float = 0.0,
+               **header_over) -> Case:
     """Meandering river: discharge hydrograph q_mean - q_amp cos(2 pi t / q_period)
     entering at the down edge (column 2), tide tide_amp sin(2 pi t / tide_period)
     at the up edge (column M), solid left/right edges.  Series are hourly
     samples, the reference's boundary format (support_funcs.cu:547-565)."""
     if axis == "i":
         hn = river_nodes(M, N, amp, wavelength, wmin, wmax, depth, noise, n_islands, rmin, rmax, seed,
-                         edge_margin=max(3, int(round(20 * min(1.0, N / 980.0))))).T.copy()
+                         edge_margin=max(3, int(round(20 * min(1.0, N / 980.0)))), open_floor=open_floor).T.copy()
     else:
         hn = river_nodes(N, M, amp, wavelength, wmin, wmax, depth, noise, n_islands, rmin, rmax, seed,
-                         edge_margin=max(3, int(round(20 * min(1.0, M / 1048.0)))))
+                         edge_margin=max(3, int(round(20 * min(1.0, M / 1048.0)))), open_floor=open_floor)
     Tmax = hours * 3600.0
     total_time = int(math.ceil(hours)) + 3
     th = np.arange(total_time, dtype=F64)
@@ -311,7 +318,8 @@ def ensemble_cases(n: int = 64, seed: int = 4, hours: float = 1.0, side_min: int
     U[0.02, 0.04] and tide amplitude ~ U[0, 1] m; dx = 20 m, dT = 5 s,
     suspended sediment from t = 0 (bed change after half an hour).  The mask
     is the 980 x 1048 river's, its meander scaled to the grid; channel widths
-    stay 30-60 cells so every discharge stays subcritical."""
+    stay 30-60 cells so every discharge stays subcritical, and the straight
+    reaches at the open edges are at least 2 m deep (tides up to 1 m)."""
     rng = np.random.default_rng(seed)
     cases = []
     for k in range(n):
@@ -323,7 +331,8 @@ def ensemble_cases(n: int = 64, seed: int = 4, hours: float = 1.0, side_min: int
         si, sj = N / 980.0, M / 1048.0
         c = river_case(N=N, M=M, hours=hours, amp=150.0 * min(1.0, sj), wavelength=400.0 * si,
                        n_islands=max(4, int(round(20 * si * sj))), n_manning=nm, q_mean=q, q_amp=0.0,
-                       tide_amp=tide, seed=1000 + k, sediment=sediment, name=f"ens{k:02d}_{N}x{M}")
+                       tide_amp=tide, seed=1000 + k, sediment=sediment, name=f"ens{k:02d}_{N}x{M}",
+                       open_floor=2.0)
         if sediment:
             c.sediment_start = 0.0
             c.bed_change_start = 1800.0

@@ -165,6 +165,50 @@ int hd_struct_ptrs(hd_ctx* c, void** arg, void** arr);
 int hd_profile(hd_ctx* c, int enable);
 int hd_kernel_times(hd_ctx* c, int n, const char** names, double* total_ms, int* launches);
 
+/* After a caller (or a surrogate) has written u, v, z: copy them into the
+ * half-step buffers t_u, t_v, t_z, set H_moi = htaiz + z on wet cells and
+ * recompute the face depths Htdu / Htdv (Htuongdoi, support_funcs.cu:338-354),
+ * so that the sediment kernels see that flow.  The mask is unchanged. */
+int hd_flow_refresh(hd_ctx* c);
+/* The sediment half of nsteps steps (boundary values, VTH / hesoK, the FS
+ * transport sweeps and FS update of each half-step, bed change when
+ * triggered: Engine.py:213-239, 331-366) on the current, frozen flow. */
+int hd_sediment_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out);
+
+/* A second context with the same case and a copy of the whole device state
+ * (fields, masks, segment and run tables, counters): for what-if runs and
+ * for tuning on a scratch copy.  src must be between whole steps. */
+int hd_clone(hd_ctx* src, hd_ctx** out);
+/* Launch-geometry autotuner: solver blocks per line (1, 2, 4) x block width
+ * of the per-row kernels (64, 128, 256), each timed over `steps` replayed
+ * steps from t on a clone of c; c keeps the fastest (its state is untouched;
+ * every choice gives identical results).  Outputs are optional. */
+int hd_autotune(hd_ctx* c, double t, const hd_step_opts* o, int steps, int* rb_mult, int* bw, double* ms_per_step);
+/* Set those two knobs directly (rb_mult 1..16, bw 64 / 128 / 256). */
+int hd_set_launch(hd_ctx* c, int rb_mult, int bw);
+
+/* Batched ensemble: independent cases (different grids, series, roughness,
+ * parameters) on one device, stepped together so that every kernel of a step
+ * is ONE launch over all of them (grid.z indexes the case; the packed wet-run
+ * tables of all cases feed one solver launch).  This has no counterpart in
+ * the reference, which runs one case per process (rep.py:145-173); it is the
+ * batching that the SURVEY's ensemble workload calls for.  Members are
+ * ordinary contexts, created, uploaded and hd_init'ed as usual and read back
+ * with hd_download; they must share the device and dT, be at the same time
+ * and have taken only whole steps (no pending half-step).  A member may not
+ * be stepped on its own while it belongs to an ensemble.  Results are
+ * identical, bit for bit, to stepping each member alone. */
+typedef struct hd_ensemble hd_ensemble;
+int hd_ensemble_create(hd_ctx* const* members, int n, hd_ensemble** out);
+int hd_ensemble_destroy(hd_ensemble* e);
+const char* hd_ensemble_last_error(const hd_ensemble* e);
+/* nsteps steps of every member from t (same options for all); counted adds
+ * each member's wet-cell count after every step to that member's counter
+ * (hd_wet_cell_steps). */
+int hd_ensemble_run(hd_ensemble* e, double t, int nsteps, const hd_step_opts* o, int counted, double* t_out);
+int hd_ensemble_profile(hd_ensemble* e, int enable);
+int hd_ensemble_kernel_times(hd_ensemble* e, int n, const char** names, double* total_ms, int* launches);
+
 /* Stand-alone kernels for unit parity. */
 /* Batched Thomas solve (UVZSolver_multithread.cu:203-228 applied per system):
  * nsys systems, system s occupies rows off[s] .. off[s]+sn[s] of the device

@@ -186,3 +186,33 @@ def test_solver_forms_identical(form, monkeypatch):
     ref.run(0.0, 30, op)
     alt.run(0.0, 30, op)
     assert_same_state(ref, alt, f"run-packed vs {form}")
+
+
+def test_clone_copies_state_and_steps_identically():
+    case = small_river()
+    g = Solver(case)
+    g.init()
+    op = case.opts()
+    t = g.run(0.0, 7, op)
+    c = g.clone()
+    assert_same_state(g, c, "clone")
+    g.run(t, 9, op)
+    c.run(t, 9, op)
+    assert_same_state(g, c, "clone after 9 more steps")
+
+
+def test_autotuned_launch_geometry_same_bits():
+    case = small_river()
+    a, b = Solver(case), Solver(case)
+    a.init()
+    b.init()
+    op = case.opts()
+    t = a.run(0.0, 5, op)
+    b.run(0.0, 5, op)
+    rb, bw, ms = a.autotune(t, op, steps=4)
+    assert rb in (1, 2, 4) and bw in (64, 128, 256) and ms > 0
+    assert_same_state(a, b, "autotune leaves the state alone")
+    b.set_launch(4, 64)
+    a.run(t, 12, op)
+    b.run(t, 12, op)
+    assert_same_state(a, b, "tuned vs other geometry")
