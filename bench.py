@@ -196,7 +196,12 @@ def main():
                "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
                          f"(FP64 restatement, OpenMP over lines, {used} threads)"}
 
-    ens = None if a.no_ensemble else ensemble_bench(a.ensemble_steps)
+    ens = None
+    if not a.no_ensemble:
+        try:
+            ens = ensemble_bench(a.ensemble_steps)
+        except Exception as e:  # reported, the river line still prints
+            ens = {"error": f"{type(e).__name__}: {e}"}
 
     line = {
         "metric": "ADI wet-cell-steps/s (FP64)",

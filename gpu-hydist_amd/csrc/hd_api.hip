@@ -60,10 +60,12 @@ struct ArrMirror {
 struct GraphKey {
     int sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode;
     double s_start;
+    int ntiles;   /* per-row kernels over this many active tiles (0: full grid) */
     bool operator<(const GraphKey& o) const
     {
-        return std::tie(sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode, s_start) <
-               std::tie(o.sed0, o.sed1, o.bed, o.count, o.kd, o.kn, o.analytic, o.ketdinh, o.qb_mode, o.s_start);
+        return std::tie(sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode, s_start, ntiles) <
+               std::tie(o.sed0, o.sed1, o.bed, o.count, o.kd, o.kn, o.analytic, o.ketdinh, o.qb_mode, o.s_start,
+                        o.ntiles);
     }
 };
 
@@ -106,6 +108,8 @@ struct hd_ctx {
     int solver = 0;
     bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
     int run_blocks_u = 0, run_blocks_v = 0;
+    int ntiles = 0;             /* active row tiles (computed at init / prepare) */
+    bool settled = false;       /* a full step has run since init / upload: land tiles are at a fixed point */
     int rb_mult = 2;            /* solver blocks per line (autotuned) */
     int bw = 256;               /* block width of the per-row full-grid kernels (autotuned) */
     Prof prof;
@@ -334,6 +338,11 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     if ((rc = dalloc(c, (void**)&d.cand_cnt, nlines * 4))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.cand, nlines * 64 * 4))) { hd_destroy(c); return rc; }
     if ((rc = dalloc(c, (void**)&d.dryflag, F))) { hd_destroy(c); return rc; }
+    {
+        size_t ncb = (size_t)(M + 3 + 63) / 64;
+        if ((rc = dalloc(c, (void**)&d.tiles, (size_t)(N + 3) * ncb * 4))) { hd_destroy(c); return rc; }
+        if ((rc = dalloc(c, (void**)&d.ntiles, 4))) { hd_destroy(c); return rc; }
+    }
     c->t_u0 = d.t_u;
     c->t_v0 = d.t_v;
     if ((rc = dalloc(c, (void**)&c->dv, 4 * sizeof(HdDev), false))) { hd_destroy(c); return rc; }
@@ -428,6 +437,7 @@ int hd_upload(hd_ctx* c, const char* name, const void* src, size_t nbytes)
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(p, src, nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->settled = false;
     return HD_OK;
 }
 
@@ -453,6 +463,42 @@ int hd_get_hmax(hd_ctx* c, double out[4])
     return HD_OK;
 }
 
+/* Active 64-column row tiles: those holding or bordering (one cell) a cell
+ * with khouot != 2.  khouot == 2 never changes after Onetime_init (wetting
+ * opens only khouot == 1 cells, drying closes only khouot == 0 cells), and
+ * outside these tiles no kernel writes a value that differs from the one
+ * already there once a full step has run. */
+static int compute_tiles(hd_ctx* c)
+{
+    int N = c->N, M = c->M, W = M + 3, ncb = (M + 3 + 63) / 64;
+    std::vector<int> kh((size_t)(N + 3) * W);
+    HIPCHK(c, hipMemcpyAsync(kh.data(), c->d.khouot, kh.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<char> live((size_t)(N + 3) * ncb, 0);
+    for (int i = 0; i < N + 3; ++i)
+        for (int j = 0; j < M + 3; ++j)
+            if (kh[(size_t)i * W + j] != 2)
+                for (int di = -1; di <= 1; ++di) {
+                    int ii = i + di;
+                    if (ii < 0 || ii > N + 2) continue;
+                    for (int dj = -1; dj <= 1; ++dj) {
+                        int jj = j + dj;
+                        if (jj < 0 || jj > M + 2) continue;
+                        live[(size_t)ii * ncb + jj / 64] = 1;
+                    }
+                }
+    std::vector<int> list;
+    for (size_t t = 0; t < live.size(); ++t)
+        if (live[t]) list.push_back((int)t);
+    int n = (int)list.size();
+    if (n) HIPCHK(c, hipMemcpyAsync(c->d.tiles, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d.ntiles, &n, 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ntiles = n;
+    c->settled = false;
+    return HD_OK;
+}
+
 /* rep.py:126-139: Onetime_init, Find_Calculation_limits_x/y, Htuongdoi,
  * preprocess_data. */
 int hd_init(hd_ctx* c)
@@ -467,6 +513,7 @@ int hd_init(hd_ctx* c)
     HIPCHK(c, hipGetLastError());
     int rc = check_flags(c);
     if (rc) return rc;
+    if ((rc = compute_tiles(c))) return rc;
     c->initialised = true;
     return HD_OK;
 }
@@ -484,7 +531,9 @@ int hd_prepare(hd_ctx* c)
     hd_launch_htuongdoi(b, c->stream);
     hd_launch_preprocess(b, c->stream);
     HIPCHK(c, hipGetLastError());
-    return check_flags(c);
+    int rc = check_flags(c);
+    if (rc) return rc;
+    return compute_tiles(c);
 }
 
 static bool bed_trigger(double t, const hd_step_opts& o)
@@ -515,8 +564,8 @@ static int check_series(hd_ctx* c, double t, const hd_step_opts& o)
  * VZ), enqueued on the stream for every context of the batch.  The device
  * time is advanced by the boundary kernel.  Returns in *slo / *eoff the row
  * range of a following bed change (rows slo .. N - eoff - 1). */
-static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sediment, int count_wet, int* slo_out,
-                         int* eoff_out)
+static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sediment, int count_wet, int ntiles,
+                         int* slo_out, int* eoff_out)
 {
     hipStream_t s = x.stream;
     bool channel = o.kenhhepd || o.kenhhepng;
@@ -533,11 +582,15 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else {
             launch(x, "adi_run_uz", [&] { hd_launch_adi_run(x.b(), 1, shift, x.run_blocks_u, s); });
         }
-        launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, s); });
+        launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
         x.swap_u();
-        launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
-        launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
-        launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, s); });
+        if (x.solver == 0) {
+            launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, s); });
+        } else {
+            launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
+            launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
+        }
+        launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, ntiles, s); });
         x.swap_v();
         launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, s); });
     } else {
@@ -551,15 +604,19 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else {
             launch(x, "adi_run_vz", [&] { hd_launch_adi_run(x.b(), 0, shift, x.run_blocks_v, s); });
         }
-        launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, s); });
+        launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
         x.swap_v();
-        launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
-        launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
-        launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, s); });
+        if (x.solver == 0) {
+            launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, s); });
+        } else {
+            launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
+            launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
+        }
+        launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, ntiles, s); });
         x.swap_u();
         launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
     }
-    launch(x, "uvz_dry_htd", [&] { hd_launch_uvz_dry(x.b(), o.kenhhepd, o.kenhhepng, x.bw, s); });
+    launch(x, "uvz_dry_htd", [&] { hd_launch_uvz_dry(x.b(), o.kenhhepd, o.kenhhepng, x.bw, ntiles, s); });
     launch(x, "segments_fs_edges", [&] { hd_launch_segments(x.b(), count_wet, s); });
     if (sediment) {
         launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), s); });
@@ -567,9 +624,14 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, which == 0 ? "fs_coef_j" : "fs_coef_i",
                    [&] { hd_launch_fs_coef(x.b(), which, o.bed_change_start, o.ketdinh, s); });
             launch(x, which == 0 ? "fs_solve_j" : "fs_solve_i", [&] { hd_launch_fs_solve(x.b(), which, s); });
-        } else {
+        } else if (x.solver == 1) {
             launch(x, which == 0 ? "fs_line_j" : "fs_line_i",
                    [&] { hd_launch_fs_line(x.b(), which, o.bed_change_start, o.ketdinh, s); });
+        } else {
+            launch(x, which == 0 ? "fs_run_j" : "fs_run_i", [&] {
+                hd_launch_fs_run(x.b(), which, o.bed_change_start, o.ketdinh,
+                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, s);
+            });
         }
         launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), s); });
         if (which == 1) { slo = 3; eoff = 1; }
@@ -587,8 +649,8 @@ static void enqueue_bed(Exec& x, const hd_step_opts& o, int slo, int eoff)
 static void enqueue_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
 {
     int slo = 2, eoff = 0;
-    enqueue_half(x, 0, o, k.sed0, 0, nullptr, nullptr);
-    enqueue_half(x, 1, o, k.sed1, k.count, &slo, &eoff);
+    enqueue_half(x, 0, o, k.sed0, 0, k.ntiles, nullptr, nullptr);
+    enqueue_half(x, 1, o, k.sed1, k.count, k.ntiles, &slo, &eoff);
     if (k.bed) enqueue_bed(x, o, slo, eoff);
 }
 
@@ -605,6 +667,7 @@ static GraphKey make_key(hd_ctx* c, double t, const hd_step_opts& o, int count)
     k.analytic = analytic_mode(o);
     k.ketdinh = o.ketdinh; k.qb_mode = o.qb_mode;
     k.s_start = o.bed_change_start;
+    k.ntiles = c->settled ? c->ntiles : 0;
     return k;
 }
 
@@ -626,6 +689,7 @@ int hd_step(hd_ctx* c, double t, const hd_step_opts* o, double* t_out)
     GraphKey k = make_key(c, t, *o, 0);
     enqueue_step(x, *o, k);
     HIPCHK(c, hipGetLastError());
+    c->settled = true;
     if (t_out) *t_out = t2;
     return HD_OK;
 }
@@ -639,7 +703,7 @@ int hd_halfstep(hd_ctx* c, int which, double t, const hd_step_opts* o)
     if (rc) return rc;
     Exec x = exec_of(c);
     set_time(x, t - c->P.dT * 0.5);
-    enqueue_half(x, which, *o, t >= o->sediment_start, 0, nullptr, nullptr);
+    enqueue_half(x, which, *o, t >= o->sediment_start, 0, 0, nullptr, nullptr);
     HIPCHK(c, hipGetLastError());
     return HD_OK;
 }
@@ -714,8 +778,10 @@ static void enqueue_sed_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
         launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, x.stream); });
         if (which == 0 ? k.sed0 : k.sed1) {
             launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), x.stream); });
-            launch(x, which == 0 ? "fs_line_j" : "fs_line_i",
-                   [&] { hd_launch_fs_line(x.b(), which, o.bed_change_start, o.ketdinh, x.stream); });
+            launch(x, which == 0 ? "fs_run_j" : "fs_run_i", [&] {
+                hd_launch_fs_run(x.b(), which, o.bed_change_start, o.ketdinh,
+                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, x.stream);
+            });
             launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), x.stream); });
             if (which == 1) { slo = 3; eoff = 1; }
         }
@@ -736,8 +802,11 @@ static int run_impl(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, doub
     Exec x = exec_of(c);
     set_time(x, t);
     std::string why;
-    hipError_t e = run_steps(x, c->graphs, t, nsteps, *o, c->P.dT, count,
-                             [&](double tt) { return make_key(c, tt, *o, count); }, why);
+    auto key = [&](double tt) { return make_key(c, tt, *o, count); };
+    int first = (!c->settled && nsteps > 0) ? 1 : 0;  /* one full-grid step, then the active tiles */
+    hipError_t e = run_steps(x, c->graphs, t, first, *o, c->P.dT, count, key, why);
+    if (e == hipSuccess && first) c->settled = true;
+    if (e == hipSuccess) e = run_steps(x, c->graphs, t, nsteps - first, *o, c->P.dT, count, key, why);
     if (e != hipSuccess) return fail(c, HD_EHIP, why + ": " + hipGetErrorString(e));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -790,6 +859,7 @@ int hd_flow_refresh(hd_ctx* c)
     hd_launch_flow_refresh(exec_of(c).b(), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->settled = false;
     return HD_OK;
 }
 
@@ -811,6 +881,8 @@ int hd_clone(hd_ctx* src, hd_ctx** out)
     }
     HIPCHK(src, hipStreamSynchronize(c->stream));
     set_knobs(c, src->rb_mult, src->bw);
+    c->ntiles = src->ntiles;
+    c->settled = src->settled;
     c->initialised = true;
     return HD_OK;
 }
@@ -1102,8 +1174,24 @@ int hd_ensemble_run(hd_ensemble* e, double t, int nsteps, const hd_step_opts* o,
            e->run_blocks_v, c0->bw, &e->prof, nullptr};
     set_time(x, t);
     std::string why;
-    hipError_t err = run_steps(x, e->graphs, t, nsteps, *o, dT, count,
-                               [&](double tt) { return make_key(c0, tt, *o, count); }, why);
+    bool settled = true;
+    int maxtiles = 0;
+    for (hd_ctx* c : e->m) {
+        settled = settled && c->settled;
+        maxtiles = std::max(maxtiles, c->ntiles);
+    }
+    auto key = [&](double tt) {
+        GraphKey k = make_key(c0, tt, *o, count);
+        k.ntiles = settled ? maxtiles : 0;
+        return k;
+    };
+    int first = (!settled && nsteps > 0) ? 1 : 0;
+    hipError_t err = run_steps(x, e->graphs, t, first, *o, dT, count, key, why);
+    if (err == hipSuccess && first) {
+        settled = true;
+        for (hd_ctx* c : e->m) c->settled = true;
+    }
+    if (err == hipSuccess) err = run_steps(x, e->graphs, t, nsteps - first, *o, dT, count, key, why);
     if (err != hipSuccess) return efail(e, HD_EHIP, why + ": " + hipGetErrorString(err));
     if ((err = hipGetLastError()) != hipSuccess) return efail(e, HD_EHIP, hipGetErrorString(err));
     if ((err = hipStreamSynchronize(e->stream)) != hipSuccess) return efail(e, HD_EHIP, hipGetErrorString(err));

@@ -240,9 +240,11 @@ __global__ void __launch_bounds__(256) k_loss(const float* __restrict__ out, con
             if (c < 3) {
                 const float pr = base[p * 3 + c] + out[p * kOutPad + c];
                 pred[p * 3 + c] = pr;
-                const float d = pr - tgt[p * 3 + c];
-                s += (double)(m * d * d);
-                g = (bf16)(2.f * m * d * inv_norm);
+                if (m != 0.f) {   /* pixels outside the mask neither add loss nor gradient */
+                    const float d = pr - tgt[p * 3 + c];
+                    s += (double)(m * d * d);
+                    g = (bf16)(2.f * m * d * inv_norm);
+                }
             }
             if (dOut) dOut[p * kOutPad + c] = g;
         }
@@ -308,13 +310,17 @@ __global__ void __launch_bounds__(256) k_expand_x(const bf16* __restrict__ xs, l
 __global__ void __launch_bounds__(256) k_pack(const HdDev* __restrict__ dv, hd_cnn_scales s, float q, float zb,
                                               bf16* __restrict__ xs, float* __restrict__ base, float* __restrict__ msk)
 {
-    const HdDev& d = dv[0];
+    const HdDevG& d = reinterpret_cast<const HdDevG*>(dv)[0];
     const long npix = (long)(d.N + 3) * (d.M + 3);
     for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += gridDim.x * 256L) {
+        /* state channels only on wet cells: elsewhere the solver's fields
+         * carry values no equation reads (they may even be non-finite) */
         const bool wet = d.khouot[p] == 0;
-        const float u = (float)(d.u[p] / s.s_u), v = (float)(d.v[p] / s.s_v), z = (float)(d.z[p] / s.s_z);
-        const float ch[HD_CNN_CIN] = {u, v, z, (float)(d.dH[p] / s.s_b), wet ? 1.f : 0.f, q / s.s_q, zb / s.s_z,
-                                      (float)(d.h[p] / s.s_h)};
+        const float u = wet ? (float)(d.u[p] / s.s_u) : 0.f;
+        const float v = wet ? (float)(d.v[p] / s.s_v) : 0.f;
+        const float z = wet ? (float)(d.z[p] / s.s_z) : 0.f;
+        const float bed = wet ? (float)(d.dH[p] / s.s_b) : 0.f;
+        const float ch[HD_CNN_CIN] = {u, v, z, bed, wet ? 1.f : 0.f, q / s.s_q, zb / s.s_z, (float)(d.h[p] / s.s_h)};
         for (int c = 0; c < HD_CNN_CIN; ++c) xs[p * HD_CNN_CIN + c] = (bf16)ch[c];
         base[p * 3 + 0] = u;
         base[p * 3 + 1] = v;
@@ -326,12 +332,13 @@ __global__ void __launch_bounds__(256) k_pack(const HdDev* __restrict__ dv, hd_c
 /* A context's u, v, z -> normalised FP32 target */
 __global__ void __launch_bounds__(256) k_target(const HdDev* __restrict__ dv, hd_cnn_scales s, float* __restrict__ tgt)
 {
-    const HdDev& d = dv[0];
+    const HdDevG& d = reinterpret_cast<const HdDevG*>(dv)[0];
     const long npix = (long)(d.N + 3) * (d.M + 3);
     for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += gridDim.x * 256L) {
-        tgt[p * 3 + 0] = (float)(d.u[p] / s.s_u);
-        tgt[p * 3 + 1] = (float)(d.v[p] / s.s_v);
-        tgt[p * 3 + 2] = (float)(d.z[p] / s.s_z);
+        const bool wet = d.khouot[p] == 0;
+        tgt[p * 3 + 0] = wet ? (float)(d.u[p] / s.s_u) : 0.f;
+        tgt[p * 3 + 1] = wet ? (float)(d.v[p] / s.s_v) : 0.f;
+        tgt[p * 3 + 2] = wet ? (float)(d.z[p] / s.s_z) : 0.f;
     }
 }
 
@@ -339,7 +346,7 @@ __global__ void __launch_bounds__(256) k_target(const HdDev* __restrict__ dv, hd
 __global__ void __launch_bounds__(256) k_unpack(const HdDev* __restrict__ dv, hd_cnn_scales s,
                                                 const float* __restrict__ out)
 {
-    const HdDev& d = dv[0];
+    const HdDevG& d = reinterpret_cast<const HdDevG*>(dv)[0];
     const int N = d.N, M = d.M, W = d.W;
     const long npix = (long)(N + 3) * W;
     for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += gridDim.x * 256L) {

@@ -23,30 +23,66 @@
 #include "../../include/hydist.h"
 #include "hd_math.h"
 
+/* Members of the device view of one context.  G qualifies the pointee of
+ * every device pointer: empty in HdDev (host code, kernel arguments), the
+ * global address space in HdDevG (in the device compile), the view kernels
+ * read from dv[blockIdx.z],
+ * so that loads and stores through these pointers are global instructions
+ * rather than flat ones (a pointer loaded from memory is otherwise generic).
+ * Layouts are identical.  Fields: the reference's Argument_Pointers arrays;
+ * hmax[4]; tmp_u / tmp_v, the smoothing ping-pong partners of t_u / t_v;
+ * AA..Bp line-major rows and factors; seg_end / seg_sn per-run side tables;
+ * runs_u / runs_v packed wet runs (line, first, last, 0) of the UZ / VZ
+ * sweeps and run_cnt their entry counts; flags [0] segment overflow, [1]
+ * boundary range; wet_count [0] run accumulator, [1] one-shot count; tdev
+ * the device time; tiles / ntiles the active 64-column row tiles; cand_cnt /
+ * cand the wetting candidates (64 per line); dryflag one byte per cell. */
+#define HD_DEV_MEMBERS(G) \
+    int N, M, W, SL, total_time, series_len; \
+    hd_params P; \
+    G double *h, *hsnham, *VISCOIDX, *u, *v, *z, *t_u, *t_v, *t_z, *Htdu, *Htdv, *H_moi, *htaiz, *htaiz_bd; \
+    G double *Kx1, *Ky1, *Tsxw, *Tsyw, *FS, *tFS, *VTH, *Kx, *Ky, *Fw, *Qbx, *Qby, *dH; \
+    G int* khouot; \
+    G double *ubt, *ubp, *vbt, *vbd, *hi; \
+    G double *bc_up, *bc_down, *bc_left, *bc_right, *CC_u, *CC_d, *CC_l, *CC_r; \
+    G int *bienQ, *boundary_type, *daui, *cuoii, *dauj, *cuoij, *moci, *mocj; \
+    G double* hmax; \
+    G double *tmp_u, *tmp_v; \
+    G double *AA, *BB, *CC, *DD, *Ap, *Bp; \
+    G double* seg_end; \
+    G int* seg_sn; \
+    G int4 *runs_u, *runs_v; \
+    G int* run_cnt; \
+    G int* flags; \
+    G long long* wet_count; \
+    G double* tdev; \
+    G int* tiles; \
+    G int* ntiles; \
+    G int *cand_cnt, *cand; \
+    G unsigned char* dryflag;
+
 struct HdDev {
-    int N, M, W, SL, total_time, series_len;
-    hd_params P;
-    /* fields */
-    double *h, *hsnham, *VISCOIDX, *u, *v, *z, *t_u, *t_v, *t_z, *Htdu, *Htdv, *H_moi, *htaiz, *htaiz_bd;
-    double *Kx1, *Ky1, *Tsxw, *Tsyw, *FS, *tFS, *VTH, *Kx, *Ky, *Fw, *Qbx, *Qby, *dH;
-    int* khouot;
-    double *ubt, *ubp, *vbt, *vbd, *hi;
-    double *bc_up, *bc_down, *bc_left, *bc_right, *CC_u, *CC_d, *CC_l, *CC_r;
-    int *bienQ, *boundary_type, *daui, *cuoii, *dauj, *cuoij, *moci, *mocj;
-    double* hmax; /* [4]: hmax_u, hmax_d, hmax_l, hmax_r */
-    /* scratch */
-    double *tmp_u, *tmp_v;          /* smoothing ping-pong partners of t_u / t_v */
-    double *AA, *BB, *CC, *DD, *Ap, *Bp;
-    double* seg_end;                /* per line*SL+seg: a2f,c2f,d2f,a2l,c2l,d2l */
-    int* seg_sn;                    /* per line*SL+seg: system size sn */
-    int4 *runs_u, *runs_v;          /* packed wet runs (line, first, last, 0) of the UZ / VZ sweeps */
-    int* run_cnt;                   /* [0]: entries of runs_u, [1]: entries of runs_v */
-    int* flags;                     /* [0]: segment overflow, [1]: boundary range */
-    long long* wet_count;
-    double* tdev;                   /* device time of the current half-step */
-    int *cand_cnt, *cand;           /* wetting candidates per line (64 slots) */
-    unsigned char* dryflag;         /* drying flag per cell */
+    HD_DEV_MEMBERS()
 };
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HD_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define HD_GLOBAL_AS
+#endif
+
+struct HdDevG {
+    HD_DEV_MEMBERS(HD_GLOBAL_AS)
+};
+
+static_assert(sizeof(HdDev) == sizeof(HdDevG), "device view layout");
+
+/* the device view of batch member blockIdx.z */
+__device__ __forceinline__ const HdDevG& dev_view(const HdDev* dv)
+{
+    return reinterpret_cast<const HdDevG*>(dv)[blockIdx.z];
+}
+
 
 /* A launch over a batch of contexts (one or an ensemble): dv points to n
  * device copies of HdDev, kernels take theirs from dv[blockIdx.z]; N, M are
@@ -67,15 +103,17 @@ void hd_launch_adi_coef(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_line(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
-void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, hipStream_t s);
+void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, int ntiles, hipStream_t s);
 void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
+void hd_launch_explicit_margins(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
-void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, hipStream_t s);
+void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
 void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s);
 void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
+void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, hipStream_t s);
 void hd_launch_update_fs(const HdBatch& b, hipStream_t s);
 void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s);
 void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s);

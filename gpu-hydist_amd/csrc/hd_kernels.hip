@@ -34,7 +34,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
  * the cells of [1,N+1]x[1,M+1] whose 4-node mean depth exceeds NANGDAY. */
 __global__ void k_onetime_init(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
@@ -66,7 +66,7 @@ __global__ void k_onetime_init(const HdDev* __restrict__ dv)
 /* Htuongdoi, support_funcs.cu:338-354 (init form, from z). */
 __global__ void k_htuongdoi(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int i = blockIdx.y + 1;
     int W = d.W;
@@ -79,7 +79,7 @@ __global__ void k_htuongdoi(const HdDev* __restrict__ dv)
 /* One edge of preprocess_data, support_funcs.cu:450-511: hi = hbar^(2/3) /
  * (sum hbar^(5/3) dX) per wet run of the edge line, summed serially in index
  * order (the reference runs it single-threaded). */
-__device__ void preprocess_edge(const HdDev& d, int edge)
+__device__ void preprocess_edge(const HdDevG& d, int edge)
 {
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
     const double* h = d.h;
@@ -110,7 +110,7 @@ __device__ void preprocess_edge(const HdDev& d, int edge)
  * ranges) in parallel, then hmax_* = 1/(ros * max htaiz_bd) per edge. */
 __global__ void k_preprocess(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int e = threadIdx.x;
     if (e < 4) preprocess_edge(d, e);
     if (e != 0) return;
@@ -138,7 +138,7 @@ __global__ void k_preprocess(const HdDev* __restrict__ dv)
 /* Boundary_value + FS_boundary for thread index i of one edge,
  * support_funcs.cu:544-617.  Only segment 0 of the edge line can match
  * (the `break` at support_funcs.cu:558). */
-__device__ void edge_value(const HdDev& d, int edge, int i, int t1, double t2)
+__device__ void edge_value(const HdDevG& d, int edge, int i, int t1, double t2)
 {
     int N = d.N, M = d.M, W = d.W, SL = d.SL, TT = d.total_time;
     const int* bt = d.boundary_type + edge * SL;
@@ -173,7 +173,7 @@ __device__ void edge_value(const HdDev& d, int edge, int i, int t1, double t2)
 
 /* boundary_up/down/left/right, support_funcs.cu:356-429 (analytic tide of the
  * narrow-channel mode), thread index i of one edge. */
-__device__ void edge_analytic(const HdDev& d, int edge, int i, double t)
+__device__ void edge_analytic(const HdDevG& d, int edge, int i, double t)
 {
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
     const double PI = d.P.PI, dX = d.P.dX, dY = d.P.dY;
@@ -211,7 +211,7 @@ __device__ void edge_analytic(const HdDev& d, int edge, int i, double t)
  * The half-step time t = t_prev + dT/2 is advanced here, on the device. */
 __global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     double* tdev = d.tdev;
     __shared__ double t_s;
     if (threadIdx.x == 0) t_s = tdev[0] + d.P.dT * 0.5;
@@ -246,7 +246,7 @@ struct Pre {
 };
 
 /* _uzSolver_calculate_preindex, UVZSolver_multithread.cu:352-407, cell (i, j). */
-__device__ Pre pre_uz(const HdDev& d, int i, int j)
+__device__ Pre pre_uz(const HdDevG& d, int i, int j)
 {
     const hd_params& P = d.P;
     int W = d.W;
@@ -284,7 +284,7 @@ __device__ Pre pre_uz(const HdDev& d, int i, int j)
 }
 
 /* _vzSolver_calculate_preindex, UVZSolver_multithread.cu:295-350, cell (i, j). */
-__device__ Pre pre_vz(const HdDev& d, int i, int j)
+__device__ Pre pre_vz(const HdDevG& d, int i, int j)
 {
     const hd_params& P = d.P;
     int W = d.W;
@@ -396,7 +396,7 @@ __device__ __forceinline__ int find_run(const int* dau, const int* cuoi, int n, 
 
 /* Wall flags of locate_segment_u (column line) / locate_segment_v (row line),
  * constant.cuh:135-178. */
-__device__ __forceinline__ void wall_flags(const HdDev& d, bool isU, int line, int first, int last, bool& br1, bool& br2)
+__device__ __forceinline__ void wall_flags(const HdDevG& d, bool isU, int line, int first, int last, bool& br1, bool& br2)
 {
     int W = d.W;
     const double* h = d.h;
@@ -418,7 +418,7 @@ __device__ __forceinline__ void wall_flags(const HdDev& d, bool isU, int line, i
 template <bool isU>
 __global__ void __launch_bounds__(256) k_adi_coef(const HdDev* __restrict__ dv, int shift)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(256) k_adi_coef(const HdDev* __restrict__ dv, 
 template <bool isU>
 __global__ void __launch_bounds__(64) k_adi_solve(const HdDev* __restrict__ dv, int shift)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > eidx) return;
@@ -684,7 +684,7 @@ __device__ __forceinline__ Row4 cell_row(const Pre& a, const Pre& b, double f4, 
 template <bool isU>
 __global__ void __launch_bounds__(64) k_adi_line(const HdDev* __restrict__ dv, int shift)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     constexpr int CH = 63;
     constexpr int kCap = 2048;
@@ -851,7 +851,7 @@ __global__ void __launch_bounds__(64) k_adi_line(const HdDev* __restrict__ dv, i
 template <bool isU>
 __global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, int shift)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     constexpr int CH = 127;
     constexpr int RW = 2 * CH;
@@ -1026,6 +1026,27 @@ __global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, in
     }
 }
 
+/* Cell (i, j) of a thread of a per-row kernel: with `tiled` the blocks walk
+ * the context's list of active 64-column row tiles (tiles that hold, or
+ * border, a cell with khouot != 2: elsewhere the per-row kernels are at a
+ * fixed point once a full step has run), else the full padded grid
+ * (blockIdx.y = row).  Returns false for threads outside the grid. */
+__device__ __forceinline__ bool row_cell(const HdDevG& d, int tiled, int& i, int& j)
+{
+    if (tiled) {
+        int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (t >= d.ntiles[0]) return false;
+        int tile = d.tiles[t];
+        int ncb = (d.M + 3 + 63) >> 6;
+        i = tile / ncb;
+        j = (tile % ncb) * 64 + (threadIdx.x & 63);
+    } else {
+        j = blockIdx.x * blockDim.x + threadIdx.x;
+        i = blockIdx.y;
+    }
+    return j < d.M + 3 && i < d.N + 3;
+}
+
 /* ================================================================ smoothing */
 
 /* Normalize + update_buffer, support_funcs.cu:682-729, writing the partner
@@ -1034,16 +1055,16 @@ __global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, in
  * (support_funcs.cu:114-127 / 173-185) for every cell, and records the
  * positions where the serial wetting pass can act (a neighbour along the
  * reset direction has khouot == 1). */
-__global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv, int isU, int post, int horizontal)
+__global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv, int isU, int post, int horizontal,
+                                                    int tiled)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int* cand_cnt = d.cand_cnt;
     int* cand = d.cand;
     unsigned char* dryflag = d.dryflag;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
     int N = d.N, M = d.M, W = d.W;
-    if (j >= M + 3 || i >= N + 3) return;
     int p = i * W + j;
     const double* val = isU ? d.t_u : d.t_v;
     double* out = isU ? d.tmp_u : d.tmp_v;
@@ -1088,7 +1109,7 @@ __global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv,
 
 /* vSolver, UVZSolver_multithread.cu:727-768, at (row, col); reads the
  * smoothed t_u. */
-__device__ double v_solver(const HdDev& d, int row, int col)
+__device__ double v_solver(const HdDevG& d, int row, int col)
 {
     const hd_params& P = d.P;
     int W = d.W;
@@ -1126,7 +1147,7 @@ __device__ double v_solver(const HdDev& d, int row, int col)
 
 /* uSolver, UVZSolver_multithread.cu:773-815, at (row, col); reads the
  * smoothed t_v. */
-__device__ double u_solver(const HdDev& d, int row, int col)
+__device__ double u_solver(const HdDevG& d, int row, int col)
 {
     const hd_params& P = d.P;
     int W = d.W;
@@ -1168,7 +1189,7 @@ __device__ double u_solver(const HdDev& d, int row, int col)
 template <bool isU>
 __global__ void __launch_bounds__(256) k_explicit(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2, eidx = isU ? d.M : d.N;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
@@ -1225,7 +1246,7 @@ __global__ void __launch_bounds__(256) k_explicit(const HdDev* __restrict__ dv)
 template <bool isU>
 __global__ void k_margins(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = 2, eidx = isU ? d.M : d.N;
     int line = sidx + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > eidx) return;
@@ -1260,6 +1281,65 @@ __global__ void k_margins(const HdDev* __restrict__ dv)
     }
 }
 
+/* solveU / solveV followed by the margins of the same line, one wave per
+ * line: the wave computes the explicit velocity of every cell first <= k <
+ * last of each wet run of the line (lanes over cells), then, after a
+ * workgroup barrier, lane 0 applies update_margin_elem_U / _V to the runs in
+ * order.  Explicit velocities read only the other direction's half-step
+ * field and the old state, and a line's margins read only that line, so this
+ * is the two-kernel order (all explicit cells, then margins) line by line. */
+template <bool isU>
+__global__ void __launch_bounds__(64) k_explicit_margins(const HdDev* __restrict__ dv)
+{
+    const HdDevG& d = dev_view(dv);
+    const int eidx = isU ? d.M : d.N;
+    const int line = 2 + blockIdx.x;
+    if (line > eidx) return;
+    const int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const int lane = threadIdx.x;
+    int nseg = isU ? d.mocj[line] : d.moci[line];
+    if (nseg > SL) nseg = SL;
+    const int* dau = (isU ? d.dauj : d.daui) + line * SL;
+    const int* cuoi = (isU ? d.cuoij : d.cuoii) + line * SL;
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int first = dau[sg], last = cuoi[sg];
+        for (int k = first + lane; k < last; k += kWave) {
+            if (isU) {
+                if (k >= 1 && k <= N) d.t_u[k * W + line] = u_solver(d, k, line);
+            } else {
+                if (k <= M && line <= N) d.t_v[line * W + k] = v_solver(d, line, k);
+            }
+        }
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    const double* h = d.h;
+    const double eps = 1e-16; /* UVZSolver_multithread.cu:12 */
+    const double ND = d.P.NANGDAY;
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int first = dau[sg], last = cuoi[sg];
+        if (isU) {
+            const int j = line;
+            bool br1 = (first > 2) || ((first == 2) && ((fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - ND) < eps));
+            bool br2 = (last < N) || ((last == N) && (fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - ND) < eps));
+            double* t = d.t_u;
+            if (br1) t[(first - 1) * W + j] = 0;
+            else t[(first - 1) * W + j] = 2 * t[first * W + j] - t[(first + 1) * W + j];
+            if (br2) t[last * W + j] = 0;
+            else t[last * W + j] = 2 * t[(last - 1) * W + j] - t[(last - 2) * W + j];
+        } else {
+            const int i = line;
+            bool br1 = (first > 2) || ((first == 2) && (fabs((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 - ND) < eps));
+            bool br2 = (last < M) || ((last == M) && (fabs((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 - ND) < eps));
+            double* t = d.t_v;
+            if (br1) t[i * W + first - 1] = 0;
+            else t[i * W + first - 1] = 2 * t[i * W + first] - t[i * W + first + 1];
+            if (br2) t[i * W + last] = 0;
+            else t[i * W + last] = 2 * t[i * W + last - 1] - t[i * W + last - 2];
+        }
+    }
+}
+
 /* ================================================================ wet / dry */
 
 /* Wetting pass of Reset_states_horizontal (support_funcs.cu:92-112) /
@@ -1270,7 +1350,7 @@ __global__ void k_margins(const HdDev* __restrict__ dv)
  * more candidates than fit the list is scanned in full. */
 __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int* cand_cnt = d.cand_cnt;
     int* cand = d.cand;
     unsigned char* dryflag = d.dryflag;
@@ -1336,16 +1416,16 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
  * and t_v at (a,b-1),(a,b); each cell applies the flags of itself and of the
  * neighbours that write it, then copies.  Face depths read the new z (t_z)
  * inside [0,N]x[0,M] and the untouched z outside, as after update_uvz. */
-__global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int kd, int kn)
+__global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int kd, int kn, int tiled)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const unsigned char* dryflag = d.dryflag;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
+    /* the run tables are rebuilt by the next kernel on the stream */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
     int N = d.N, M = d.M, W = d.W;
-    if (j >= M + 3 || i >= N + 3) return;
     int p = i * W + j;
-    if (p == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; } /* the run tables are rebuilt next */
     bool me = dryflag[p];
     bool dn = (i + 1 <= N + 2) ? dryflag[p + W] : false;
     bool rt = (j + 1 <= M + 2) ? dryflag[p + 1] : false;
@@ -1374,7 +1454,7 @@ __global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int k
  * support_funcs.cu:338-354). */
 __global__ void __launch_bounds__(256) k_flow_refresh(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
@@ -1396,7 +1476,7 @@ __global__ void __launch_bounds__(256) k_flow_refresh(const HdDev* __restrict__ 
 
 /* Interpolate_FS_ng, support_funcs.cu:189-208, one column j of the row
  * `location` (source row clamped to [0, N+1]; zero divisor -> quotient 0). */
-__device__ void fs_ng(const HdDev& d, int location, int offset, int sign, int j)
+__device__ void fs_ng(const HdDevG& d, int location, int offset, int sign, int j)
 {
     int W = d.W;
     if (d.t_u[location * W + j] > 0) {
@@ -1413,7 +1493,7 @@ __device__ void fs_ng(const HdDev& d, int location, int offset, int sign, int j)
 
 /* Interpolate_FS_d, support_funcs.cu:210-229, one row i of column `location`
  * (source column clamped to [0, M+1]). */
-__device__ void fs_d(const HdDev& d, int location, int offset, int sign, int i)
+__device__ void fs_d(const HdDevG& d, int location, int offset, int sign, int i)
 {
     int W = d.W;
     if (d.t_v[i * W + location] > 0) {
@@ -1430,7 +1510,7 @@ __device__ void fs_d(const HdDev& d, int location, int offset, int sign, int i)
 
 __global__ void k_reset_runs(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     if (threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
 }
 
@@ -1490,7 +1570,7 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
 /* Interpolate_FS_ng rows N and 2, then Interpolate_FS_d column M, by one
  * workgroup (the two reference kernels' order; the column reads the rows'
  * corner results). */
-__device__ void fs_edges_block(const HdDev& d)
+__device__ void fs_edges_block(const HdDevG& d)
 {
     int N = d.N, M = d.M;
     for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
@@ -1516,7 +1596,7 @@ __device__ void fs_edges_block(const HdDev& d)
  * [nrowb, nrowb + ncolb): columns 2 + 4b' ... */
 __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     if (blockIdx.x == (unsigned)(nrowb + ncolb)) { /* the FS edge interpolations (independent of the mask) */
         fs_edges_block(d);
         return;
@@ -1573,7 +1653,7 @@ __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ d
 
 __global__ void k_wet_count(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int i = blockIdx.x;
     int lane = threadIdx.x;
     if (i >= d.N + 3) return;
@@ -1679,9 +1759,17 @@ void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s)
     else hipLaunchKernelGGL(k_adi_solve<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
-void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, hipStream_t s)
+/* per-row kernels: the full grid, or ntiles active tiles (bw/64 per block) */
+static inline dim3 row_grid(const HdBatch& b, int bw, int ntiles)
 {
-    hipLaunchKernelGGL(k_normalize, grid2d(b, bw), dim3(bw), 0, s, b.dv, isU, post, horizontal);
+    if (ntiles > 0) return dim3((ntiles + (bw >> 6) - 1) / (bw >> 6), 1, b.n);
+    return grid2d(b, bw);
+}
+
+void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, int ntiles, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_normalize, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, isU, post, horizontal,
+                       ntiles > 0 ? 1 : 0);
 }
 
 void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s)
@@ -1699,15 +1787,22 @@ void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s)
     else hipLaunchKernelGGL(k_margins<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
+void hd_launch_explicit_margins(const HdBatch& b, int isU, hipStream_t s)
+{
+    int nl = (isU ? b.M : b.N) - 1;
+    if (isU) hipLaunchKernelGGL(k_explicit_margins<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv);
+    else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv);
+}
+
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)
 {
     int n = (horizontal ? b.N : b.M) + 1;
     hipLaunchKernelGGL(k_wetting, dim3((n + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, horizontal);
 }
 
-void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, hipStream_t s)
+void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_uvz, grid2d(b, bw), dim3(bw), 0, s, b.dv, kd, kn);
+    hipLaunchKernelGGL(k_uvz, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, kd, kn, ntiles > 0 ? 1 : 0);
 }
 
 void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s)

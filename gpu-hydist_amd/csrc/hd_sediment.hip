@@ -41,7 +41,7 @@ __device__ double source_nchs(const hd_params& P, double wsm, double Zf, double 
 /* Find_VTH (Sediment_Transport.cu:541-568) and hesoK (:508-539). */
 __global__ void __launch_bounds__(256) k_vth_hesok(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     int i = blockIdx.y;
     int N = d.N, M = d.M, W = d.W;
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) k_vth_hesok(const HdDev* __restrict__ dv)
     }
 }
 
-__device__ __forceinline__ void wall_flags2(const HdDev& d, bool isJ, int line, int first, int last, bool& br1, bool& br2)
+__device__ __forceinline__ void wall_flags2(const HdDevG& d, bool isJ, int line, int first, int last, bool& br1, bool& br2)
 {
     int W = d.W;
     const double* h = d.h;
@@ -93,7 +93,7 @@ struct FsRow {
  * gamav before Zf and && at the last row, as written) or
  * _FSi_calculate__mactrix_coeff (:62-165, row lines). */
 template <bool isJ>
-__device__ FsRow fs_row(const HdDev& d, double t, double s_start, int ketdinh, int i, int j, int first, int last,
+__device__ FsRow fs_row(const HdDevG& d, double t, double s_start, int ketdinh, int i, int j, int first, int last,
                         bool br1, bool br2)
 {
     const hd_params& P = d.P;
@@ -165,7 +165,7 @@ __device__ FsRow fs_row(const HdDev& d, double t, double s_start, int ketdinh, i
 template <bool isJ>
 __global__ void __launch_bounds__(256) k_fs_coef(const HdDev* __restrict__ dv, double s_start, int ketdinh)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const double* tdev = d.tdev;
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(256) k_fs_coef(const HdDev* __restrict__ dv, d
 template <bool isJ>
 __global__ void __launch_bounds__(64) k_fs_solve(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
     int line = 3 + blockIdx.x * blockDim.x + threadIdx.x;
     if (line > (isJ ? M - 1 : N - 1)) return;
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(64) k_fs_solve(const HdDev* __restrict__ dv)
 template <bool isJ>
 __global__ void __launch_bounds__(64) k_fs_line(const HdDev* __restrict__ dv, double s_start, int ketdinh)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const double* tdev = d.tdev;
     constexpr int CH = 64;
     constexpr int kCap = 2048;
@@ -381,10 +381,126 @@ __global__ void __launch_bounds__(64) k_fs_line(const HdDev* __restrict__ dv, do
     }
 }
 
+/* The FS sweep with one wave per wet run from the packed run tables (runs_u
+ * for the column sweep FSj, runs_v for the row sweep FSi), lines 3 .. M-1 /
+ * N-1 as the reference launches them.  Rows are assembled by the 64 lanes in
+ * chunks of 256 (four per lane) into LDS; lane 0 carries the Thomas
+ * recurrence (the same operations in the same order as k_fs_line and the
+ * reference's tridiagSolver); factors of all but the last chunk go to the
+ * run's slice of Ap/Bp.  The end copies use the clamped first / last
+ * solutions kept in LDS. */
+template <bool isJ>
+__global__ void __launch_bounds__(64) k_fs_run(const HdDev* __restrict__ dv, double s_start, int ketdinh)
+{
+    const HdDevG& d = dev_view(dv);
+    constexpr int RW = 256;
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];
+    __shared__ double sV[2]; /* clamped x of the first and last row */
+    const int N = d.N, M = d.M, W = d.W;
+    const int lane = threadIdx.x;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const int lmax = isJ ? M - 1 : N - 1;
+    const int cap = (isJ ? M + 3 : N + 3) * d.SL;
+    const int cnt = d.run_cnt[isJ ? 0 : 1] < cap ? d.run_cnt[isJ ? 0 : 1] : cap;
+    const int4* runs = isJ ? d.runs_u : d.runs_v;
+    const double t = d.tdev[0];
+    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+        const int4 rn = runs[ri];
+        const int line = rn.x, first = rn.y, last = rn.z;
+        if (line < 3 || line > lmax || last < first + 2) continue;
+        auto at = [&](int k) -> size_t { return isJ ? (size_t)k * W + line : (size_t)line * W + k; };
+        bool br1, br2;
+        wall_flags2(d, isJ, line, first, last, br1, br2);
+        const int sn = last - first - 2;
+        const int k0 = first + 1;
+        double* gAp = d.Ap + (size_t)line * tw + k0;
+        double* gBp = d.Bp + (size_t)line * tw + k0;
+        const int nch = sn / RW + 1;
+        double ap = 0, bp = 0, xs = 0;
+        for (int c = 0; c < nch; ++c) {
+            const int rb = RW * c;
+            const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+            for (int r = rb + lane; r <= re; r += 64) {
+                const int k = k0 + r;
+                const int i = isJ ? k : line, j = isJ ? line : k;
+                FsRow w = fs_row<isJ>(d, t, s_start, ketdinh, i, j, first, last, br1, br2);
+                sA[r - rb] = w.A; sB[r - rb] = w.B; sC[r - rb] = w.C; sD[r - rb] = w.D;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int q = 0;
+                if (rb == 0) {
+                    if (sn == 0) {
+                        xs = sD[0] / sB[0];
+                    } else {
+                        ap = -sC[0] / sB[0];
+                        bp = sD[0] / sB[0];
+                        sA[0] = ap; sB[0] = bp;
+                    }
+                    q = 1;
+                }
+                const int qmid = ((re < sn - 1) ? re : sn - 1) - rb;
+#pragma unroll 4
+                for (; q <= qmid; ++q) {
+                    double A = sA[q], B = sB[q], C = sC[q], D = sD[q];
+                    double e = A * ap + B;
+                    double nap = -C / e;
+                    double nbp = (D - (A * bp)) / e;
+                    ap = nap;
+                    bp = nbp;
+                    sA[q] = ap; sB[q] = bp;
+                }
+                if (re == sn && sn > 0) {
+                    double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
+                    xs = (D - (A * bp)) / (B + (A * ap));
+                }
+            }
+            __syncthreads();
+            if (c + 1 < nch)
+                for (int q = lane; q <= re - rb; q += 64) { gAp[rb + q] = sA[q]; gBp[rb + q] = sB[q]; }
+        }
+        for (int c = nch - 1; c >= 0; --c) {
+            const int rb = RW * c;
+            const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+            if (c + 1 < nch) {
+                __syncthreads();
+                for (int q = lane; q <= re - rb; q += 64) { sA[q] = gAp[rb + q]; sB[q] = gBp[rb + q]; }
+                __syncthreads();
+            }
+            if (lane == 0) {
+                int q = re - rb;
+                if (c == nch - 1) { sD[q] = xs; --q; }
+#pragma unroll 4
+                for (; q >= 0; --q) {
+                    xs = sB[q] + (sA[q] * xs);
+                    sD[q] = xs;
+                }
+            }
+            __syncthreads();
+            for (int q = lane; q <= re - rb; q += 64) {
+                double v = sD[q];
+                if (v < 0) v = d.P.NDnen;
+                d.tFS[at(k0 + rb + q)] = v;
+                if (rb + q == 0) sV[0] = v;
+                if (rb + q == sn) sV[1] = v;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const double* tqq = isJ ? d.t_u : d.t_v;
+            if ((br1) || (tqq[at(first)] == 0)) d.tFS[at(first)] = sV[0];
+            else d.tFS[at(first)] = d.FS[at(first)];
+            if ((br2) || (tqq[at(last)] == 0)) d.tFS[at(last)] = sV[1];
+            else d.tFS[at(last)] = d.FS[at(last)];
+        }
+        __syncthreads();
+    }
+}
+
 /* Update_FS, Sediment_Transport.cu:620-629. */
 __global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     int i = blockIdx.y + 1;
     if (i > d.N || j > d.M) return;
@@ -395,7 +511,7 @@ __global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv)
  * `!VTH` test as written, FIXED uses VTH != 0; corner copies braced). */
 __global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, int qb_mode)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
     int i = blockIdx.y + 2;
     int N = d.N, M = d.M, W = d.W;
@@ -439,7 +555,7 @@ __global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, i
  * reads t_u[1], as written). */
 __global__ void __launch_bounds__(256) k_bedload(const HdDev* __restrict__ dv, int ketdinh, int slo, int eoff)
 {
-    const HdDev& d = dv[blockIdx.z];
+    const HdDevG& d = dev_view(dv);
     const int sidx = slo, eidx = d.N - eoff;
     __shared__ int s_moc;
     __shared__ int s_dau[24], s_cuoi[24];
@@ -547,4 +663,10 @@ void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh,
     if (nl <= 0) return;
     if (which == 0) hipLaunchKernelGGL(k_fs_line<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
     else hipLaunchKernelGGL(k_fs_line<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
+}
+
+void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, hipStream_t s)
+{
+    if (which == 0) hipLaunchKernelGGL(k_fs_run<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
+    else hipLaunchKernelGGL(k_fs_run<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
 }

@@ -199,7 +199,8 @@ def river_nodes(N: int, M: int, amp: float, wavelength: float, wmin: float, wmax
     `edge_margin` cells from the open edges j = 0 and j = M-1.  With
     open_floor > 0 the straight reaches next to the open edges get that
     minimum depth across the whole channel width (steep banks), so a large
-    tide does not dry and re-wet cells on the open boundary itself."""
+    tide does not dry and re-wet cells on the open boundary itself, and the
+    islands stay out of those reaches."""
     rng = np.random.default_rng(seed)
     j = np.arange(M, dtype=F64)[None, :]
     i = np.arange(N, dtype=F64)[:, None]
@@ -228,8 +229,10 @@ def river_nodes(N: int, M: int, amp: float, wavelength: float, wmin: float, wmax
         hn = np.where(reach, np.maximum(hn, open_floor), hn)
     jj = np.arange(M)[None, :]
     ii = np.arange(N)[:, None]
+    # with open_floor the islands also stay out of the straight open-edge reaches
+    margin = max(edge_margin, taper) if open_floor > 0.0 else edge_margin
     for _ in range(n_islands):
-        j0 = rng.uniform(edge_margin + rmax, M - 1 - edge_margin - rmax)
+        j0 = rng.uniform(margin + rmax, M - 1 - margin - rmax)
         xd = min(max((min(j0, (M - 1) - j0) - 0.5 * taper) / (0.5 * taper), 0.0), 1.0)
         ic0 = N / 2.0 + amp * xd * xd * (3.0 - 2.0 * xd) * math.sin(k * j0)
         w0 = 0.5 * (wmin + wmax) + 0.5 * (wmax - wmin) * math.sin(2.0 * math.pi * j0 / (0.73 * wavelength) + phase)
@@ -311,7 +314,7 @@ def config(n: int, scale: float = 1.0, hours: Optional[float] = None, **kw) -> C
 
 
 def ensemble_cases(n: int = 64, seed: int = 4, hours: float = 1.0, side_min: int = 256, side_max: int = 1024,
-                   sediment: bool = True):
+                   sediment: bool = True, tide_max: float = 1.0):
     """The batched-ensemble workload: `n` independent meandering rivers with
     grid sides N, M drawn uniformly from [side_min, side_max] and, per
     scenario, a constant discharge Q ~ U[500, 5000] m3/s, Manning n ~
@@ -327,12 +330,14 @@ def ensemble_cases(n: int = 64, seed: int = 4, hours: float = 1.0, side_min: int
         M = int(rng.integers(side_min, side_max + 1))
         q = float(rng.uniform(500.0, 5000.0))
         nm = float(rng.uniform(0.02, 0.04))
-        tide = float(rng.uniform(0.0, 1.0))
+        tide = float(rng.uniform(0.0, 1.0)) * tide_max
         si, sj = N / 980.0, M / 1048.0
-        c = river_case(N=N, M=M, hours=hours, amp=150.0 * min(1.0, sj), wavelength=400.0 * si,
+        # meander amplitude and wavelength scale together (the 980 x 1048
+        # river's bend slope), the amplitude also bounded by the width
+        c = river_case(N=N, M=M, hours=hours, amp=150.0 * min(si, sj), wavelength=400.0 * si,
                        n_islands=max(4, int(round(20 * si * sj))), n_manning=nm, q_mean=q, q_amp=0.0,
                        tide_amp=tide, seed=1000 + k, sediment=sediment, name=f"ens{k:02d}_{N}x{M}",
-                       open_floor=2.0)
+                       open_floor=2.0, SL=24)
         if sediment:
             c.sediment_start = 0.0
             c.bed_change_start = 1800.0

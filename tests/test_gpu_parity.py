@@ -171,11 +171,12 @@ def test_full_river_bitwise():
     assert_same_state(o, g, "river 980x1048, 4 steps")
 
 
-@pytest.mark.parametrize("form", ["line", "rows"])
-def test_solver_forms_identical(form, monkeypatch):
-    """The three hydrodynamic solver forms (one wave per run, one wave per
-    line, rows kernel + lane per line) give identical bits."""
-    case = small_river()
+@pytest.mark.parametrize("form,make", [("line", small_river), ("rows", small_river),
+                                       ("line", small_river_sediment), ("rows", small_river_sediment)])
+def test_solver_forms_identical(form, make, monkeypatch):
+    """The three solver forms (one wave per run, one wave per line, rows
+    kernel + lane per line), hydrodynamics and sediment, give identical bits."""
+    case = make()
     ref = Solver(case)
     monkeypatch.setenv("HYDIST_SOLVER", form)
     alt = Solver(case)
