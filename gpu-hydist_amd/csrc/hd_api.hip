@@ -108,6 +108,7 @@ struct hd_ctx {
     int solver = 0;
     bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
     int run_blocks_u = 0, run_blocks_v = 0;
+    double* bed_snap = nullptr; /* dH snapshot for hd_bed_extrapolate (allocated on first use) */
     int ntiles = 0;             /* active row tiles (computed at init / prepare) */
     bool settled = false;       /* a full step has run since init / upload: land tiles are at a fixed point */
     int rb_mult = 2;            /* solver blocks per line (autotuned) */
@@ -850,6 +851,30 @@ int hd_sediment_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, doub
     return check_flags(c);
 }
 
+int hd_bed_snapshot(hd_ctx* c)
+{
+    if (!c) return HD_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t F = (size_t)(c->N + 3) * (c->M + 3);
+    if (!c->bed_snap) {
+        int rc = dalloc(c, (void**)&c->bed_snap, F * 8, false);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->bed_snap, c->d.dH, F * 8, hipMemcpyDeviceToDevice, c->stream));
+    return HD_OK;
+}
+
+int hd_bed_extrapolate(hd_ctx* c, double factor)
+{
+    if (!c) return HD_EINVAL;
+    if (!c->bed_snap) return fail(c, HD_ESTATE, "hd_bed_extrapolate before hd_bed_snapshot");
+    HIPCHK(c, hipSetDevice(c->device));
+    hd_launch_bed_extrapolate(exec_of(c).b(), c->bed_snap, factor, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HD_OK;
+}
+
 int hd_flow_refresh(hd_ctx* c)
 {
     if (!c) return HD_EINVAL;
@@ -860,6 +885,28 @@ int hd_flow_refresh(hd_ctx* c)
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->settled = false;
+    return HD_OK;
+}
+
+int hd_copy_state(hd_ctx* dst, hd_ctx* src)
+{
+    if (!dst || !src) return HD_EINVAL;
+    if (dst->N != src->N || dst->M != src->M || dst->SL != src->SL || dst->series_len != src->series_len ||
+        dst->total_time != src->total_time || dst->state_allocs.size() != src->state_allocs.size())
+        return fail(dst, HD_ESHAPE, "hd_copy_state: contexts of different cases");
+    if (src->state() != 0 || dst->state() != 0) return fail(dst, HD_ESTATE, "hd_copy_state between half-steps");
+    HIPCHK(dst, hipSetDevice(dst->device));
+    HIPCHK(dst, hipStreamSynchronize(src->stream));
+    for (size_t k = 0; k < dst->state_allocs.size(); ++k) {
+        if (dst->state_allocs[k].second != src->state_allocs[k].second)
+            return fail(dst, HD_ESHAPE, "hd_copy_state: layout");
+        HIPCHK(dst, hipMemcpyAsync(dst->state_allocs[k].first, src->state_allocs[k].first, src->state_allocs[k].second,
+                                   hipMemcpyDeviceToDevice, dst->stream));
+    }
+    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    dst->ntiles = src->ntiles;
+    dst->settled = src->settled;
+    dst->initialised = src->initialised;
     return HD_OK;
 }
 

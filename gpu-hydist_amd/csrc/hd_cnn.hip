@@ -228,20 +228,21 @@ __global__ void __launch_bounds__(256) k_relu_grad(const float* __restrict__ dA,
  * dOut (bf16, kOutPad channels) = 2 m (pred - tgt) * inv_norm. */
 __global__ void __launch_bounds__(256) k_loss(const float* __restrict__ out, const float* __restrict__ base,
                                               const float* __restrict__ tgt, const float* __restrict__ msk,
-                                              long npix, float inv_norm, float* __restrict__ pred,
+                                              long npix, float inv_norm, hd_cnn_scales sc, float* __restrict__ pred,
                                               bf16* __restrict__ dOut, double* __restrict__ loss)
 {
     __shared__ double red[256];
     double s = 0;
+    const float dsc[3] = {sc.d_u, sc.d_v, sc.d_z};
     for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += gridDim.x * 256L) {
         const float m = msk[p];
         for (int c = 0; c < kOutPad; ++c) {
             bf16 g = (bf16)0.f;
             if (c < 3) {
-                const float pr = base[p * 3 + c] + out[p * kOutPad + c];
+                const float pr = base[p * 3 + c] + dsc[c] * out[p * kOutPad + c];
                 pred[p * 3 + c] = pr;
                 if (m != 0.f) {   /* pixels outside the mask neither add loss nor gradient */
-                    const float d = pr - tgt[p * 3 + c];
+                    const float d = (pr - tgt[p * 3 + c]) / dsc[c];   /* increment error in d units */
                     s += (double)(m * d * d);
                     g = (bf16)(2.f * m * d * inv_norm);
                 }
@@ -321,7 +322,9 @@ __global__ void __launch_bounds__(256) k_pack(const HdDev* __restrict__ dv, hd_c
         const float z = wet ? (float)(d.z[p] / s.s_z) : 0.f;
         const float bed = wet ? (float)(d.dH[p] / s.s_b) : 0.f;
         const float ch[HD_CNN_CIN] = {u, v, z, bed, wet ? 1.f : 0.f, q / s.s_q, zb / s.s_z, (float)(d.h[p] / s.s_h)};
-        for (int c = 0; c < HD_CNN_CIN; ++c) xs[p * HD_CNN_CIN + c] = (bf16)ch[c];
+        /* the network sees every channel clipped to [-8, 8]; the residual
+         * base keeps the unclipped u, v, z */
+        for (int c = 0; c < HD_CNN_CIN; ++c) xs[p * HD_CNN_CIN + c] = (bf16)fminf(fmaxf(ch[c], -8.f), 8.f);
         base[p * 3 + 0] = u;
         base[p * 3 + 1] = v;
         base[p * 3 + 2] = z;
@@ -352,9 +355,9 @@ __global__ void __launch_bounds__(256) k_unpack(const HdDev* __restrict__ dv, hd
     for (long p = blockIdx.x * 256L + threadIdx.x; p < npix; p += gridDim.x * 256L) {
         const int i = (int)(p / W), j = (int)(p % W);
         if (i < 1 || i > N || j < 1 || j > M || d.khouot[p] != 0) continue;
-        d.u[p] += (double)s.s_u * (double)out[p * kOutPad + 0];
-        d.v[p] += (double)s.s_v * (double)out[p * kOutPad + 1];
-        d.z[p] += (double)s.s_z * (double)out[p * kOutPad + 2];
+        d.u[p] += (double)s.s_u * (double)s.d_u * (double)out[p * kOutPad + 0];
+        d.v[p] += (double)s.s_v * (double)s.d_v * (double)out[p * kOutPad + 1];
+        d.z[p] += (double)s.s_z * (double)s.d_z * (double)out[p * kOutPad + 2];
     }
 }
 
@@ -389,7 +392,7 @@ struct hd_cnn {
     int H = 0, W = 0, B = 0, L = 0;
     long P = 0;   /* pixels per image */
     std::vector<Layer> layer;
-    hd_cnn_scales sc{1, 1, 1, 1, 1, 1};
+    hd_cnn_scales sc{1, 1, 1, 1, 1, 1, 1, 1, 1};
     std::vector<void*> allocs;
     bf16* x0 = nullptr;                 /* [B][P][32] */
     float* base = nullptr;              /* [B][P][3] */
@@ -490,7 +493,7 @@ static int loss_and_grad(hd_cnn* n, int nimg, bool grad, double* loss)
     CHK(n, hipMemsetAsync(n->loss, 0, sizeof(double), n->stream));
     float inv = n->nmask > 0 ? 1.f / (3.f * n->nmask) : 0.f;
     hipLaunchKernelGGL(k_loss, dim3(nblk(npix) < 2048 ? nblk(npix) : 2048), dim3(256), 0, n->stream, n->out, n->base,
-                       n->tgt, n->msk, npix, inv, n->pred, grad ? n->dZ : nullptr, n->loss);
+                       n->tgt, n->msk, npix, inv, n->sc, n->pred, grad ? n->dZ : nullptr, n->loss);
     CHK(n, hipGetLastError());
     if (loss) {
         CHK(n, hipMemcpyAsync(loss, n->loss, sizeof(double), hipMemcpyDeviceToHost, n->stream));
@@ -622,7 +625,7 @@ int hd_cnn_destroy(hd_cnn* n)
 
 int hd_cnn_set_scales(hd_cnn* n, const hd_cnn_scales* s)
 {
-    if (!n || !s) return HD_EINVAL;
+    if (!n || !s || !(s->d_u > 0) || !(s->d_v > 0) || !(s->d_z > 0)) return HD_EINVAL;
     n->sc = *s;
     return HD_OK;
 }

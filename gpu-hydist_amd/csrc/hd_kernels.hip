@@ -154,6 +154,17 @@ __device__ void edge_value(const HdDevG& d, int edge, int i, int t1, double t2)
     else { isU = true; location = N; loc_ext = N + 1; moc = d.moci; dau = d.daui; cuoi = d.cuoii; bc = d.bc_right; CCs = d.CC_r; hi = d.hi + 2 * (N + 3) + (M + 3); barr = d.ubp; }
     if (!(moc[location] > 0 && dau[location * SL] <= i && i <= cuoi[location * SL])) return;
     const int seg_no = 0;
+    /* a column edge (0, 1) leaves its cell of row 2 / N to the row edge
+     * (2 / 3) that writes the same field there: in the reference's edge
+     * order the later edge wins */
+    bool yield = false;
+    if (!isU && (i == 2 || i == N)) {
+        int re = (i == 2) ? 2 : 3;
+        const int* rbt = d.boundary_type + re * SL;
+        int rloc = (i == 2) ? 2 : N;
+        bool covers = d.moci[rloc] > 0 && d.daui[rloc * SL] <= location && location <= d.cuoii[rloc * SL];
+        yield = covers && ((rbt[0] == 2) == (bt[seg_no] == 2));
+    }
     double bv = bc[seg_no * TT + t1] * (1.0 - t2) + bc[seg_no * TT + t1 + 1] * t2;
     if (bt[seg_no] == 2) {
         barr[i] = bv * hi[i];
@@ -161,10 +172,10 @@ __device__ void edge_value(const HdDevG& d, int edge, int i, int t1, double t2)
         d.t_z[location * W + i] = bv;
         d.t_z[loc_ext * W + i] = bv;
     } else {
-        d.t_z[i * W + location] = bv;
+        if (!yield) d.t_z[i * W + location] = bv;
         d.t_z[i * W + loc_ext] = bv;
     }
-    if (bt[seg_no] != 2) return;
+    if (bt[seg_no] != 2 || yield) return;
     int fi, fj;
     if (isU) { fi = location; fj = i; } else { fj = location; fi = i; }
     double cv = CCs[seg_no * TT + t1] * (1.0 - t2) + CCs[seg_no * TT + t1 + 1] * t2;
@@ -233,10 +244,18 @@ __global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
         if (threadIdx.x == 0) d.flags[1] = 1;
         return;
     }
-    for (int e = 0; e < 4; ++e) {
-        for (int i = 2 + threadIdx.x; i <= imax; i += blockDim.x) edge_value(d, e, i, t1, t2);
-        __syncthreads();
-    }
+    /* the four edges at once, 256 threads each, over the edge line's first
+     * wet run (the only one the reference updates); corner cells go to the
+     * later edge by the rule in edge_value */
+    const int e = threadIdx.x >> 8, lt = threadIdx.x & 255;
+    const int SL = d.SL;
+    const int* moc = e < 2 ? d.mocj : d.moci;
+    const int* dau = e < 2 ? d.dauj : d.daui;
+    const int* cuoi = e < 2 ? d.cuoij : d.cuoii;
+    const int location = e == 0 ? d.M : (e == 3 ? d.N : 2);
+    if (moc[location] <= 0) return;
+    const int a = dau[location * SL], b = cuoi[location * SL];
+    for (int i = (a > 2 ? a : 2) + lt; i <= b && i <= imax; i += 256) edge_value(d, e, i, t1, t2);
 }
 
 /* ================================================================ ADI rows */
@@ -1472,6 +1491,17 @@ __global__ void __launch_bounds__(256) k_flow_refresh(const HdDev* __restrict__ 
     }
 }
 
+/* dH += f * (dH - snap) over the padded grid (bed-change extrapolation) */
+__global__ void __launch_bounds__(256) k_bed_extrapolate(const HdDev* __restrict__ dv, const double* snap, double f)
+{
+    const HdDevG& d = dev_view(dv);
+    const long n = (long)(d.N + 3) * d.W;
+    for (long p = blockIdx.x * 256L + threadIdx.x; p < n; p += gridDim.x * 256L) {
+        double x = d.dH[p];
+        d.dH[p] = x + f * (x - snap[p]);
+    }
+}
+
 /* ================================================================ segments */
 
 /* Interpolate_FS_ng, support_funcs.cu:189-208, one column j of the row
@@ -1820,4 +1850,11 @@ int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double
 {
     hipLaunchKernelGGL(k_thomas_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap, Bp);
     return 0;
+}
+
+void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, hipStream_t s)
+{
+    long n = (long)(b.N + 3) * (b.M + 3);
+    long g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_bed_extrapolate, dim3((unsigned)(g < 4096 ? g : 4096), 1, b.n), dim3(256), 0, s, b.dv, snap, f);
 }

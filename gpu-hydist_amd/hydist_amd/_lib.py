@@ -24,7 +24,7 @@ EXPORTS = (
     "hd_wet_count", "hd_sync", "hd_struct_ptrs", "hd_profile", "hd_kernel_times", "hd_thomas_batch",
     "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
-    "hd_clone", "hd_autotune", "hd_set_launch",
+    "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
 )
 
 _lib = None
@@ -77,6 +77,9 @@ def load() -> ctypes.CDLL:
         "hd_clone": (ci, [vp, ctypes.POINTER(vp)]),
         "hd_autotune": (ci, [vp, cd, po, ci, ip, ip, dp]),
         "hd_set_launch": (ci, [vp, ci, ci]),
+        "hd_bed_snapshot": (ci, [vp]),
+        "hd_copy_state": (ci, [vp, vp]),
+        "hd_bed_extrapolate": (ci, [vp, cd]),
         "hd_sediment_run": (ci, [vp, cd, ci, po, dp]),
         "hd_ensemble_create": (ci, [ctypes.POINTER(vp), ci, ctypes.POINTER(vp)]),
         "hd_ensemble_destroy": (ci, [vp]),
@@ -202,6 +205,10 @@ class Solver:
         other.h = h
         return other
 
+    def copy_state_from(self, other):
+        """Overwrite this context's device state with other's (same case)."""
+        self._chk(load().hd_copy_state(self.h, other.h))
+
     def autotune(self, t, opts, steps=10):
         """Pick the launch geometry on a clone; returns (rb_mult, bw, ms/step)."""
         rb, bw, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
@@ -211,6 +218,12 @@ class Solver:
 
     def set_launch(self, rb_mult, bw):
         self._chk(load().hd_set_launch(self.h, rb_mult, bw))
+
+    def bed_snapshot(self):
+        self._chk(load().hd_bed_snapshot(self.h))
+
+    def bed_extrapolate(self, factor):
+        self._chk(load().hd_bed_extrapolate(self.h, float(factor)))
 
     def flow_refresh(self):
         self._chk(load().hd_flow_refresh(self.h))

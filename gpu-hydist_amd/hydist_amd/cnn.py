@@ -23,7 +23,7 @@ EXPORTS = (
 
 
 class Scales(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_float) for n in ("s_u", "s_v", "s_z", "s_b", "s_q", "s_h")]
+    _fields_ = [(n, ctypes.c_float) for n in ("s_u", "s_v", "s_z", "s_b", "s_q", "s_h", "d_u", "d_v", "d_z")]
 
 
 _declared = False
@@ -108,6 +108,7 @@ class Network:
             pass
 
     def set_scales(self, **kw):
+        kw = {"d_u": 1.0, "d_v": 1.0, "d_z": 1.0, **kw}
         s = Scales(**{k: float(v) for k, v in kw.items()})
         self._chk(lib().hd_cnn_set_scales(self.h, ctypes.byref(s)))
 

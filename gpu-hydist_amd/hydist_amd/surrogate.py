@@ -51,7 +51,7 @@ def forcing(case, t: float):
     return series_at(a["bc_down"], t), series_at(a["bc_up"], t)
 
 
-SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=0.01, s_q=3000.0, s_h=10.0)
+SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=1.0, s_q=3000.0, s_h=10.0, d_u=0.02, d_v=0.02, d_z=0.01)
 
 
 def river_variant(k: int, scale: float = 1.0, hours: float = 30.0, seed: int = 50):
@@ -147,12 +147,21 @@ class RunReport:
     adi_steps: int = 0
 
 
-def plain_run(case, t0: float, t1: float, g: Solver) -> RunReport:
+def plain_run(case, t0: float, t1: float, g: Solver, log=None) -> RunReport:
+    """The plain FP64 path over [t0, t1] (replayed graphs, one simulated day
+    per call)."""
     op = case.opts()
     n = int(round((t1 - t0) / case.dT))
+    day = int(round(86400.0 / case.dT))
     g.sync()
     w0 = time.perf_counter()
-    g.run(t0, n, op)
+    t, done = t0, 0
+    while done < n:
+        k = min(day, n - done)
+        t = g.run(t, k, op)
+        done += k
+        if log:
+            log(f"    plain ADI: {done * case.dT / 3600.0:.0f} h")
     g.sync()
     w = time.perf_counter() - w0
     return RunReport((t1 - t0) / 3600.0, w, (t1 - t0) / 3600.0 / w, adi_steps=n)
@@ -165,9 +174,15 @@ def wet_volume(g: Solver) -> float:
 
 
 def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: int, check_every: int = 10,
-                  z_bound: float = 5e-3, mass_bound: float = 1e-4, fallback_macro: int = 5) -> RunReport:
+                  z_bound: float = 5e-3, mass_bound: float = 1e-4, fallback_macro: int = 5,
+                  morph_substeps: int = 0) -> RunReport:
     """Bed-change phase from t0 to t1 with the network standing in for the
-    hydrodynamics (module docstring)."""
+    hydrodynamics (module docstring).  The last state that passed a check is
+    kept on the device (a clone); a failed check restores it and runs plain
+    ADI from there for `fallback_macro` macro steps.  morph_substeps = m > 0:
+    of the K sediment steps on the predicted flow only m run and the bed
+    change of the last of them is repeated K - m times (morphological
+    acceleration over the quasi-steady flow, hd_bed_extrapolate)."""
     op = case.opts()
     rep = RunReport(0, 0, 0)
     t = t0
@@ -175,6 +190,8 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
     adi_left = 0
     g.sync()
     w0 = time.perf_counter()
+    good = g.clone()          # last verified state
+    t_good = t
     while t < t1 - 1e-9:
         k = min(K, int(round((t1 - t) / case.dT)))
         rep.macro_steps += 1
@@ -182,12 +199,24 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
             t = g.run(t, k, op)
             rep.adi_steps += k
             adi_left = max(0, adi_left - 1)
+            if adi_left == 0:
+                good.copy_state_from(g)
+                t_good = t
             continue
         q, zb = forcing(case, t + K * case.dT)
         net.predict_into(g, q, zb)
         rep.surrogate_calls += 1
         calls_since_check += 1
-        t = g.sediment_run(t, K, op)
+        if morph_substeps > 0:
+            ts = t
+            if morph_substeps > 1:
+                t = g.sediment_run(t, morph_substeps - 1, op)
+            g.bed_snapshot()
+            g.sediment_run(t, 1, op)
+            g.bed_extrapolate(K - morph_substeps)
+            t = ts + K * case.dT
+        else:
+            t = g.sediment_run(t, K, op)
         if calls_since_check >= check_every:
             calls_since_check = 0
             rep.checks += 1
@@ -200,10 +229,16 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
             m = (kh == 0) & (g.get("khouot") == 0)
             dz = float(np.abs(z1[m] - z0[m]).max()) if m.any() else 0.0
             dv = abs(wet_volume(g) - v0) / max(v0, 1e-30)
-            if dz > z_bound or dv > mass_bound:
+            if dz <= z_bound and dv <= mass_bound:     # NaN fails too
+                good.copy_state_from(g)
+                t_good = t
+            else:
                 rep.fallbacks += 1
+                g.copy_state_from(good)
+                t = t_good
                 adi_left = fallback_macro
     g.sync()
+    good.close()
     rep.wall_s = time.perf_counter() - w0
     rep.sim_hours = (t - t0) / 3600.0
     rep.sim_hours_per_wall_s = rep.sim_hours / rep.wall_s
@@ -211,7 +246,7 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
 
 
 def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 3, batch: int = 4,
-             lr: float = 1e-3, days: float = 7.0, layers: int = 8, log=print) -> dict:
+             lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
     bed-change run with the plain one on the config-3 river (bed change every
     step, from hour 2): simulated hours per wall second of both, and the bed
@@ -236,27 +271,32 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
     g1 = Solver(case)
     g1.init()
     g1.run(0.0, int(round(tb / case.dT)), op)
-    plain = plain_run(case, tb, te, g1)
+    plain = plain_run(case, tb, te, g1, log=log)
     log(f"  plain ADI: {plain.sim_hours:.1f} h in {plain.wall_s:.1f} s")
-    g2 = Solver(case)
-    g2.init()
-    g2.run(0.0, int(round(tb / case.dT)), op)
-    sur = surrogate_run(case, tb, te, g2, net, K)
-    log(f"  surrogate: {sur.sim_hours:.1f} h in {sur.wall_s:.1f} s, {sur.surrogate_calls} calls, "
-        f"{sur.fallbacks} fallbacks")
-    d1, d2, kh = g1.get("dH"), g2.get("dH"), g1.get("khouot")
+    d1, kh = g1.get("dH"), g1.get("khouot")
     wet = kh == 0
-    bed_err = float(np.abs(d1[wet] - d2[wet]).max())
+    modes = {}
+    for label, m in (("full_sediment_steps", 0), (f"morph_{morph}", morph)):
+        g2 = Solver(case)
+        g2.init()
+        g2.run(0.0, int(round(tb / case.dT)), op)
+        sur = surrogate_run(case, tb, te, g2, net, K, morph_substeps=m)
+        d2 = g2.get("dH")
+        err = float(np.abs(d1[wet] - d2[wet]).max())
+        rms = float(np.sqrt(np.mean((d1[wet] - d2[wet]) ** 2)))
+        log(f"  surrogate ({label}): {sur.sim_hours:.1f} h in {sur.wall_s:.1f} s, {sur.surrogate_calls} calls, "
+            f"{sur.fallbacks} fallbacks, bed error max {err:.3e} m rms {rms:.3e} m")
+        modes[label] = {"sim_hours_per_wall_s": sur.sim_hours_per_wall_s,
+                        "speedup": sur.sim_hours_per_wall_s / plain.sim_hours_per_wall_s,
+                        "bed_error_max_m": err, "bed_error_rms_m": rms, "surrogate_calls": sur.surrogate_calls,
+                        "checks": sur.checks, "fallbacks": sur.fallbacks, "adi_steps": sur.adi_steps}
+        g2.close()
     bed_change = float(np.abs(d1[wet]).max())
     net.close()
     return {
         "workload": f"{case.name} bed change {days:g} d (every step), K={K}",
         "plain_sim_hours_per_wall_s": plain.sim_hours_per_wall_s,
-        "surrogate_sim_hours_per_wall_s": sur.sim_hours_per_wall_s,
-        "speedup": sur.sim_hours_per_wall_s / plain.sim_hours_per_wall_s,
-        "bed_error_max_m": bed_err, "bed_change_max_m": bed_change,
-        "surrogate_calls": sur.surrogate_calls, "checks": sur.checks, "fallbacks": sur.fallbacks,
-        "adi_steps_in_surrogate_run": sur.adi_steps,
+        "bed_change_max_m": bed_change, "modes": modes,
         "train": {"samples": n, "epochs": epochs, "loss": hist, "val_loss": val,
                   "val_loss_persistence": pers, "gen_s": gen_s, "train_s": train_s},
     }
@@ -274,7 +314,8 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=3)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--days", type=float, default=7.0)
+    ap.add_argument("--morph", type=int, default=6)
     a = ap.parse_args()
     r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
-                 days=a.days, log=lambda s: print(s, flush=True))
+                 days=a.days, morph=a.morph, log=lambda s: print(s, flush=True))
     print(json.dumps(r))

@@ -175,10 +175,20 @@ int hd_flow_refresh(hd_ctx* c);
  * triggered: Engine.py:213-239, 331-366) on the current, frozen flow. */
 int hd_sediment_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out);
 
+/* Bed-change extrapolation (morphological acceleration over a frozen flow):
+ * hd_bed_snapshot keeps a copy of dH; hd_bed_extrapolate then sets
+ * dH += factor * (dH - snapshot), i.e. repeats the bed change made since the
+ * snapshot `factor` more times. */
+int hd_bed_snapshot(hd_ctx* c);
+int hd_bed_extrapolate(hd_ctx* c, double factor);
+
 /* A second context with the same case and a copy of the whole device state
  * (fields, masks, segment and run tables, counters): for what-if runs and
  * for tuning on a scratch copy.  src must be between whole steps. */
 int hd_clone(hd_ctx* src, hd_ctx** out);
+/* Copy the whole device state of src into dst, a context of the same case
+ * (a clone); both between whole steps. */
+int hd_copy_state(hd_ctx* dst, hd_ctx* src);
 /* Launch-geometry autotuner: solver blocks per line (1, 2, 4) x block width
  * of the per-row kernels (64, 128, 256), each timed over `steps` replayed
  * steps from t on a clone of c; c keeps the fastest (its state is untouched;

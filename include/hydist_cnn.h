@@ -39,10 +39,14 @@ extern "C" {
 typedef struct hd_cnn hd_cnn;
 
 /* Input channels, in order: u/s_u, v/s_v, z/s_z, dH/s_b, wet mask, discharge
- * forcing q(t)/s_q, water-level forcing zb(t)/s_z, depth h/s_h.  Scales are
+ * forcing q(t)/s_q, water-level forcing zb(t)/s_z, depth h/s_h (each clipped
+ * to [-8, 8]).  The network's three outputs are the increments of u/s_u,
+ * v/s_v, z/s_z over the horizon in units of d_u, d_v, d_z, and the loss is
+ * the masked mean square of the increment error in those units.  Scales are
  * set once (hd_cnn_set_scales). */
 typedef struct hd_cnn_scales {
     float s_u, s_v, s_z, s_b, s_q, s_h;
+    float d_u, d_v, d_z;
 } hd_cnn_scales;
 
 /* A network for images of H x W pixels, up to `batch` images per call,
