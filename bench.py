@@ -6,10 +6,12 @@ A step is one full ADI step (UZ + VZ half-steps) of the whole river; the
 timed region replays K captured steps (hd_run) with the state resident in
 HBM.  One JSON line is printed with the throughput, the roofline of the
 dominant kernel (per-launch HIP-event durations recorded on the library's
-stream inside this process) and the CPU checker timed on a bounded sample on
-this host's cores.
+stream inside this process; HBM traffic per launch from the rocprofv3
+counter runs recorded in profiles/traffic.json), the CPU checker timed on a
+bounded sample on this host's cores, and the batched-ensemble workload (64
+seeded rivers, one launch per kernel over all of them).
 
-    python bench.py [--gpus 1] [--steps 720] [--warmup 20]
+    python bench.py [--gpus 1] [--steps 720] [--warmup 20] [--no-cpu] [--no-ensemble]
 """
 from __future__ import annotations
 
@@ -66,6 +68,17 @@ def cpu_threads() -> int:
     except AttributeError:
         n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(case, budget_s, threads):
@@ -210,7 +223,8 @@ def main():
         rate, n, dt, used = cpu_baseline(case, a.cpu_seconds, cpu_threads())
         cpu = {"value": rate, "unit": "wet-cell-steps/s", "cores": used, "kind": "port",
                "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
-                         f"(FP64 restatement, OpenMP over lines, {used} threads)"}
+                         f"(FP64 restatement, OpenMP over lines, {used} threads of {os.cpu_count()} "
+                         f"logical CPUs, {cpu_model()})"}
 
     ens = None
     if not a.no_ensemble:

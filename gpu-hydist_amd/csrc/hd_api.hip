@@ -109,6 +109,7 @@ struct hd_ctx {
     bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
     int run_blocks_u = 0, run_blocks_v = 0;
     double* bed_snap = nullptr; /* dH snapshot for hd_bed_extrapolate (allocated on first use) */
+    double* z_snap = nullptr;   /* z snapshot for hd_z_change (allocated on first use), + 3 result slots */
     int ntiles = 0;             /* active row tiles (computed at init / prepare) */
     bool settled = false;       /* a full step has run since init / upload: land tiles are at a fixed point */
     int rb_mult = 2;            /* solver blocks per line (autotuned) */
@@ -872,6 +873,40 @@ int hd_bed_extrapolate(hd_ctx* c, double factor)
     hd_launch_bed_extrapolate(exec_of(c).b(), c->bed_snap, factor, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HD_OK;
+}
+
+int hd_z_snapshot(hd_ctx* c)
+{
+    if (!c) return HD_EINVAL;
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t F = (size_t)(c->N + 3) * (c->M + 3);
+    if (!c->z_snap) {
+        int rc = dalloc(c, (void**)&c->z_snap, (F + 3) * 8, false);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->z_snap, c->d.z, F * 8, hipMemcpyDeviceToDevice, c->stream));
+    return HD_OK;
+}
+
+int hd_z_change(hd_ctx* c, double* max_abs_dz, double* wet_volume, double* wet_volume_snap)
+{
+    if (!c) return HD_EINVAL;
+    if (!c->z_snap) return fail(c, HD_ESTATE, "hd_z_change before hd_z_snapshot");
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t F = (size_t)(c->N + 3) * (c->M + 3);
+    double* res = c->z_snap + F;
+    double zero[3] = {0, 0, 0};
+    HIPCHK(c, hipMemcpyAsync(res, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    hd_launch_z_change(exec_of(c).b(), c->z_snap, res, c->stream);
+    HIPCHK(c, hipGetLastError());
+    double out[3];
+    HIPCHK(c, hipMemcpyAsync(out, res, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    /* cell volumes are per unit area: the sums of depths */
+    if (max_abs_dz) *max_abs_dz = out[0];
+    if (wet_volume) *wet_volume = out[1];
+    if (wet_volume_snap) *wet_volume_snap = out[2];
     return HD_OK;
 }
 

@@ -1373,6 +1373,10 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
     int* cand_cnt = d.cand_cnt;
     int* cand = d.cand;
     unsigned char* dryflag = d.dryflag;
+    /* each lane's candidate list lives in LDS (a private array would sit in
+     * scratch memory, one global round trip per sort step) */
+    __shared__ int s_list[64][kCand + 1];
+    int* list = s_list[threadIdx.x];
     int line = blockIdx.x * blockDim.x + threadIdx.x;
     int N = d.N, M = d.M, W = d.W;
     int lo = horizontal ? 1 : 1, hi = horizontal ? N : M;
@@ -1380,7 +1384,6 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
     int n = cand_cnt[line];
     if (n == 0) return;
     cand_cnt[line] = 0;
-    int list[kCand];
     bool full = n > kCand;
     int cnt = full ? 0 : n;
     if (!full) {
@@ -1502,6 +1505,48 @@ __global__ void __launch_bounds__(256) k_bed_extrapolate(const HdDev* __restrict
     }
 }
 
+/* Over the wet cells (khouot == 0) of [1,N]x[1,M]: max |z - zref| into
+ * out[0] (as the bit pattern of a non-negative double, compared as an
+ * unsigned integer), sum of htaiz + z into out[1], sum of htaiz + zref into
+ * out[2] (FP64 atomics; the sums' last bits depend on the order). */
+__global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, const double* zref, double* out)
+{
+    const HdDevG& d = dev_view(dv);
+    const int N = d.N, M = d.M, W = d.W;
+    __shared__ double s1[256], s2[256];
+    __shared__ unsigned long long smx[256];
+    double v1 = 0, v2 = 0;
+    unsigned long long mx = 0;
+    const long n = (long)(N + 3) * W;
+    for (long p = blockIdx.x * 256L + threadIdx.x; p < n; p += gridDim.x * 256L) {
+        int i = (int)(p / W), j = (int)(p % W);
+        if (i < 1 || i > N || j < 1 || j > M || d.khouot[p] != 0) continue;
+        double dz = fabs(d.z[p] - zref[p]);
+        unsigned long long b = __double_as_longlong(dz);
+        if (!(dz <= 1e300)) b = 0x7FF0000000000000ull; /* NaN counts as infinite */
+        if (b > mx) mx = b;
+        v1 += d.htaiz[p] + d.z[p];
+        v2 += d.htaiz[p] + zref[p];
+    }
+    s1[threadIdx.x] = v1;
+    s2[threadIdx.x] = v2;
+    smx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            s1[threadIdx.x] += s1[threadIdx.x + k];
+            s2[threadIdx.x] += s2[threadIdx.x + k];
+            if (smx[threadIdx.x + k] > smx[threadIdx.x]) smx[threadIdx.x] = smx[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax((unsigned long long*)out, smx[0]);
+        atomicAdd(out + 1, s1[0]);
+        atomicAdd(out + 2, s2[0]);
+    }
+}
+
 /* ================================================================ segments */
 
 /* Interpolate_FS_ng, support_funcs.cu:189-208, one column j of the row
@@ -1545,7 +1590,7 @@ __global__ void k_reset_runs(const HdDev* __restrict__ dv)
 }
 
 /* Wet runs of one line (a row when ROW, else a column), positions 2..L, by
- * one wave: the line is read four 64-cell chunks at a time (all loads of a
+ * one wave: the line is read eight 64-cell chunks at a time (all loads of a
  * batch in flight together), neighbours come from lane shuffles, starts and
  * ends are compacted with ballots.  The rule is the serial scan's
  * (support_funcs.cu:252-279): a run must start at <= L-2; it ends at its last
@@ -1562,18 +1607,19 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
     const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
     int ns = 0, ne = 0;
     int carry = 0; /* wet(first position of the chunk - 1) */
-    for (int b0 = 2; b0 <= L; b0 += 4 * kWave) {
-        int w[4];
+    constexpr int kB = 8; /* chunks of 64 cells per batch, all loads in flight together */
+    for (int b0 = 2; b0 <= L; b0 += kB * kWave) {
+        int w[kB];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w[u] = wetv(b0 + kWave * u + lane);
-        const int after = wetv(b0 + 4 * kWave); /* the cell after the batch */
+        for (int u = 0; u < kB; ++u) w[u] = wetv(b0 + kWave * u + lane);
+        const int after = wetv(b0 + kB * kWave); /* the cell after the batch */
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kB; ++u) {
             if (b0 + kWave * u > L) break;
             const int x = b0 + kWave * u + lane;
             const int up = __shfl_up(w[u], 1, kWave);
             const int dn = __shfl_down(w[u], 1, kWave);
-            const int nx0 = (u < 3) ? __shfl(w[u < 3 ? u + 1 : u], 0, kWave) : after;
+            const int nx0 = (u < kB - 1) ? __shfl(w[u < kB - 1 ? u + 1 : u], 0, kWave) : after;
             const int last_of_chunk = __shfl(w[u], kWave - 1, kWave);
             const int prevw = lane == 0 ? carry : up;
             const int nextw = lane == kWave - 1 ? nx0 : dn;
@@ -1857,4 +1903,11 @@ void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, h
     long n = (long)(b.N + 3) * (b.M + 3);
     long g = (n + 255) / 256;
     hipLaunchKernelGGL(k_bed_extrapolate, dim3((unsigned)(g < 4096 ? g : 4096), 1, b.n), dim3(256), 0, s, b.dv, snap, f);
+}
+
+void hd_launch_z_change(const HdBatch& b, const double* zref, double* out, hipStream_t s)
+{
+    long n = (long)(b.N + 3) * (b.M + 3);
+    long g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_z_change, dim3((unsigned)(g < 1024 ? g : 1024), 1, b.n), dim3(256), 0, s, b.dv, zref, out);
 }

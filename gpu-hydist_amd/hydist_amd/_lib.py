@@ -25,6 +25,7 @@ EXPORTS = (
     "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
     "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
+    "hd_z_snapshot", "hd_z_change",
 )
 
 _lib = None
@@ -79,6 +80,8 @@ def load() -> ctypes.CDLL:
         "hd_set_launch": (ci, [vp, ci, ci]),
         "hd_bed_snapshot": (ci, [vp]),
         "hd_copy_state": (ci, [vp, vp]),
+        "hd_z_snapshot": (ci, [vp]),
+        "hd_z_change": (ci, [vp, dp, dp, dp]),
         "hd_bed_extrapolate": (ci, [vp, cd]),
         "hd_sediment_run": (ci, [vp, cd, ci, po, dp]),
         "hd_ensemble_create": (ci, [ctypes.POINTER(vp), ci, ctypes.POINTER(vp)]),
@@ -204,6 +207,15 @@ class Solver:
         other.N, other.M, other._params, other.case = self.N, self.M, self._params, self.case
         other.h = h
         return other
+
+    def z_snapshot(self):
+        self._chk(load().hd_z_snapshot(self.h))
+
+    def z_change(self):
+        """(max |z - z_snapshot| on wet cells, wet volume now, at the snapshot)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(load().hd_z_change(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
     def copy_state_from(self, other):
         """Overwrite this context's device state with other's (same case)."""

@@ -220,15 +220,11 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
         if calls_since_check >= check_every:
             calls_since_check = 0
             rep.checks += 1
-            z0 = g.get("z")
-            v0 = wet_volume(g)
-            kh = g.get("khouot")
+            g.z_snapshot()
             t = g.run(t, 1, op)
             rep.adi_steps += 1
-            z1 = g.get("z")
-            m = (kh == 0) & (g.get("khouot") == 0)
-            dz = float(np.abs(z1[m] - z0[m]).max()) if m.any() else 0.0
-            dv = abs(wet_volume(g) - v0) / max(v0, 1e-30)
+            dz, v1, v0 = g.z_change()
+            dv = abs(v1 - v0) / max(abs(v0), 1e-30)
             if dz <= z_bound and dv <= mass_bound:     # NaN fails too
                 good.copy_state_from(g)
                 t_good = t

@@ -182,6 +182,13 @@ int hd_sediment_run(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, doub
 int hd_bed_snapshot(hd_ctx* c);
 int hd_bed_extrapolate(hd_ctx* c, double factor);
 
+/* Water-level check of a step from a saved state: hd_z_snapshot keeps a copy
+ * of z; hd_z_change returns, over the wet cells of [1,N]x[1,M], the largest
+ * |z - z_saved| (a NaN counts as infinite) and the sums of htaiz + z now and
+ * at the snapshot (the wet volume per unit cell area). */
+int hd_z_snapshot(hd_ctx* c);
+int hd_z_change(hd_ctx* c, double* max_abs_dz, double* wet_volume, double* wet_volume_snap);
+
 /* A second context with the same case and a copy of the whole device state
  * (fields, masks, segment and run tables, counters): for what-if runs and
  * for tuning on a scratch copy.  src must be between whole steps. */
