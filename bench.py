@@ -34,8 +34,26 @@ B_STEP_SED = 304           # ... with suspended sediment
 KERNEL_BYTES = {
     # fused assembly + Thomas + extraction of one half-step: u, v, z, Htdu,
     # Htdv, H_moi, Kx1|Ky1, VISCOIDX, Tsxw|Tsyw read, t_z and t_u|t_v written
+    "adi_spec_uz": 9 * 8 + 2 * 8,
+    "adi_spec_vz": 9 * 8 + 2 * 8,
+    # the same work with the serial Thomas on one lane (HYDIST_SOLVER=run)
     "adi_run_uz": 9 * 8 + 2 * 8,
     "adi_run_vz": 9 * 8 + 2 * 8,
+    # explicit v (u): v, u, t_u, Htdv, Ky1, H_moi, z, VISCOIDX, Tsyw read
+    # (mirror for u), t_v (t_u) written
+    "explicit_margins_v": 9 * 8 + 8,
+    "explicit_margins_u": 9 * 8 + 8,
+    # state copy + drying + face depths: t_z, t_u, t_v, H_moi, h read;
+    # z, u, v, Htdu, Htdv written
+    "uvz_dry_htd": 5 * 8 + 5 * 8,
+    # smoothing: t_u (t_v) and khouot read, the partner buffer written; the
+    # second pass also reads htaiz, t_z and writes H_moi
+    "smooth_u": 8 + 4 + 8,
+    "smooth_v": 8 + 4 + 8,
+    "smooth_v_hmoi": 8 + 4 + 8 + 2 * 8 + 8,
+    "smooth_u_hmoi": 8 + 4 + 8 + 2 * 8 + 8,
+    # wet-run tables: khouot read
+    "segments_fs_edges": 4,
     # the same work, one wave per line (HYDIST_SOLVER=line)
     "adi_line_uz": 9 * 8 + 2 * 8,
     "adi_line_vz": 9 * 8 + 2 * 8,

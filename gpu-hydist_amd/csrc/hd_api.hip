@@ -102,9 +102,10 @@ struct hd_ctx {
     std::string err;
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::map<GraphKey, hipGraphExec_t> sed_graphs;   /* sediment-only steps (hd_sediment_run) */
-    /* hydrodynamic solver form (HYDIST_SOLVER): 0 = one wave per wet run
-     * (default), 1 = one wave per line ("line"), 2 = rows kernel + one lane
-     * per line ("rows").  All three give identical bits. */
+    /* hydrodynamic solver form (HYDIST_SOLVER): 0 = one wave per wet run,
+     * speculative parallel Thomas (default), 3 = one wave per wet run, serial
+     * Thomas on one lane ("run"), 1 = one wave per line ("line"), 2 = rows
+     * kernel + one lane per line ("rows").  All four give identical bits. */
     int solver = 0;
     bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
     int run_blocks_u = 0, run_blocks_v = 0;
@@ -273,7 +274,7 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
     {
         const char* sv = getenv("HYDIST_SOLVER");
         c->solver_v1 = sv && std::string(sv) == "rows";
-        c->solver = c->solver_v1 ? 2 : (sv && std::string(sv) == "line") ? 1 : 0;
+        c->solver = c->solver_v1 ? 2 : (sv && std::string(sv) == "line") ? 1 : (sv && std::string(sv) == "run") ? 3 : 0;
     }
     int rc;
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, HD_EHIP, "hipSetDevice failed"); }
@@ -581,12 +582,14 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, "adi_solve_uz", [&] { hd_launch_adi_solve(x.b(), 1, shift, s); });
         } else if (x.solver == 1) {
             launch(x, "adi_line_uz", [&] { hd_launch_adi_line(x.b(), 1, shift, s); });
-        } else {
+        } else if (x.solver == 3) {
             launch(x, "adi_run_uz", [&] { hd_launch_adi_run(x.b(), 1, shift, x.run_blocks_u, s); });
+        } else {
+            launch(x, "adi_spec_uz", [&] { hd_launch_adi_spec(x.b(), 1, shift, x.run_blocks_u, s); });
         }
         launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
         x.swap_u();
-        if (x.solver == 0) {
+        if (x.solver == 0 || x.solver == 3) {
             launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, s); });
         } else {
             launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
@@ -603,12 +606,14 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, "adi_solve_vz", [&] { hd_launch_adi_solve(x.b(), 0, shift, s); });
         } else if (x.solver == 1) {
             launch(x, "adi_line_vz", [&] { hd_launch_adi_line(x.b(), 0, shift, s); });
-        } else {
+        } else if (x.solver == 3) {
             launch(x, "adi_run_vz", [&] { hd_launch_adi_run(x.b(), 0, shift, x.run_blocks_v, s); });
+        } else {
+            launch(x, "adi_spec_vz", [&] { hd_launch_adi_spec(x.b(), 0, shift, x.run_blocks_v, s); });
         }
         launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
         x.swap_v();
-        if (x.solver == 0) {
+        if (x.solver == 0 || x.solver == 3) {
             launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, s); });
         } else {
             launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
@@ -630,9 +635,9 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, which == 0 ? "fs_line_j" : "fs_line_i",
                    [&] { hd_launch_fs_line(x.b(), which, o.bed_change_start, o.ketdinh, s); });
         } else {
-            launch(x, which == 0 ? "fs_run_j" : "fs_run_i", [&] {
+            launch(x, which == 0 ? (x.solver == 0 ? "fs_spec_j" : "fs_run_j") : (x.solver == 0 ? "fs_spec_i" : "fs_run_i"), [&] {
                 hd_launch_fs_run(x.b(), which, o.bed_change_start, o.ketdinh,
-                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, s);
+                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, x.solver == 0, s);
             });
         }
         launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), s); });
@@ -780,9 +785,9 @@ static void enqueue_sed_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
         launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, x.stream); });
         if (which == 0 ? k.sed0 : k.sed1) {
             launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), x.stream); });
-            launch(x, which == 0 ? "fs_run_j" : "fs_run_i", [&] {
+            launch(x, which == 0 ? (x.solver == 0 ? "fs_spec_j" : "fs_run_j") : (x.solver == 0 ? "fs_spec_i" : "fs_run_i"), [&] {
                 hd_launch_fs_run(x.b(), which, o.bed_change_start, o.ketdinh,
-                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, x.stream);
+                                 which == 0 ? x.run_blocks_u : x.run_blocks_v, x.solver == 0, x.stream);
             });
             launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), x.stream); });
             if (which == 1) { slo = 3; eoff = 1; }

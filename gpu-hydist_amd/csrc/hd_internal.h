@@ -103,6 +103,7 @@ void hd_launch_adi_coef(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_line(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
+void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
 void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, int ntiles, hipStream_t s);
 void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
@@ -113,7 +114,8 @@ void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s);
 void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
-void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, hipStream_t s);
+void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, int spec,
+                      hipStream_t s);
 void hd_launch_update_fs(const HdBatch& b, hipStream_t s);
 void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s);
 void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s);

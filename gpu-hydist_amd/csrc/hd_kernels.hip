@@ -14,8 +14,11 @@
  * so the bits match the CPU checker.  Wave size is 64.
  */
 #include "hd_internal.h"
+#include "hd_thomas.h"
 
 #define W_ (d.W)
+
+using namespace hd_spec;
 
 namespace {
 
@@ -1045,6 +1048,185 @@ __global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, in
     }
 }
 
+__device__ __forceinline__ Pre shfl_pre(const Pre& a, int src)
+{
+    Pre b;
+    b.f1 = __shfl(a.f1, src, kWave); b.f2 = __shfl(a.f2, src, kWave); b.f3 = __shfl(a.f3, src, kWave);
+    b.f5 = __shfl(a.f5, src, kWave); b.a2 = __shfl(a.a2, src, kWave); b.c2 = __shfl(a.c2, src, kWave);
+    b.d2 = __shfl(a.d2, src, kWave);
+    return b;
+}
+
+/* Fused UZ / VZ solve of one wet run per wave with the speculative parallel
+ * Thomas above: the same rows as k_adi_run (_calculate_preindex,
+ * _calculate_abcd, _calculate_matrix_coeff, tridiag, _extract_solution,
+ * UVZSolver_multithread.cu:203-723) and the same bits.  Chunks of 255 cells
+ * (510 rows) are assembled by the 64 lanes, four cells each (cell t = lane +
+ * 64 h; the preindex of cell t + 1 comes from the neighbouring lane by a
+ * shuffle), into LDS; the forward sweep and the back substitution run over
+ * the chunk on all lanes; factors of all but the last chunk go to the run's
+ * slice of the Ap/Bp line buffers and come back for back substitution. */
+template <bool isU>
+__global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, int shift)
+{
+    const HdDevG& d = dev_view(dv);
+    const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
+    constexpr int CH = 255; /* cells per LDS chunk (the rows of cell t need cells t and t + 1) */
+    constexpr int RW = 2 * CH;
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];  /* rows; sA holds x after the forward sweep */
+    __shared__ double sAp[RW], sBp[RW];
+    __shared__ double sE[6];
+    __shared__ double sX[3]; /* X(0), X(sn), X(last interior u row) */
+    const int lane = threadIdx.x;
+    const int N = d.N, M = d.M, W = d.W;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const int cap = (isU ? M + 3 : N + 3) * d.SL;
+    const int cnt = d.run_cnt[isU ? 0 : 1] < cap ? d.run_cnt[isU ? 0 : 1] : cap;
+    const int4* runs = isU ? d.runs_u : d.runs_v;
+    const bool dkQ1 = isU ? d.bienQ[2] != 0 : d.bienQ[1] != 0;
+    const bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
+    const int dkfr = isU ? N : M;
+    const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
+    double* tz = d.t_z;
+    double* tq = isU ? d.t_u : d.t_v;
+    const int nbr = (lane + 1) & (kWave - 1);
+    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+        const int4 rn = runs[ri];
+        const int line = rn.x, first = rn.y, last = rn.z;
+        if (line < sidx || line > eidx) continue;
+        auto at = [&](int k) -> size_t { return isU ? (size_t)k * W + line : (size_t)line * W + k; };
+        bool br1, br2;
+        wall_flags(d, isU, line, first, last, br1, br2);
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
+        const double ub1 = isU ? d.ubt[line] : d.vbd[line];
+        const double ub2 = isU ? d.ubp[line] : d.vbt[line];
+        const double TZ_l = tz[at(first)];
+        const double TZ_r = tz[at(last)];
+        const int ru = (skind != S_OPEN) ? 2 * (last - 1 - first) + 1 : 2 * (last - 1 - first);
+        double* gAp = d.Ap + (size_t)line * tw + 2 * first;
+        double* gBp = d.Bp + (size_t)line * tw + 2 * first;
+        if (sn >= 0) {
+            const int nch = sn / RW + 1;
+            double ap = 0, bp = 0, xs = 0;
+            for (int c = 0; c < nch; ++c) {
+                const int c0 = first + CH * c;
+                const int rb = RW * c;
+                const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+                Pre pa[4] = {};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    int k = c0 + lane + kWave * h;
+                    if (k <= last) {
+                        Pre a = isU ? pre_uz(d, k, line) : pre_vz(d, line, k);
+                        pa[h] = a;
+                        if (k == first) { sE[0] = a.a2; sE[1] = a.c2; sE[2] = a.d2; }
+                        if (k == last) { sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2; }
+                    }
+                }
+                Pre nb[4];
+#pragma unroll
+                for (int h = 0; h < 4; ++h) nb[h] = shfl_pre(pa[h], nbr);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    int t = lane + kWave * h;
+                    int k = c0 + t;
+                    if (t < CH && k <= last) {
+                        const Pre& a = pa[h];
+                        Pre b = a;
+                        if (k + 1 <= last) b = (lane < kWave - 1) ? nb[h] : nb[h < 3 ? h + 1 : 3];
+                        for (int e = 0; e < 2; ++e) {
+                            int rr = 2 * (k - first) + e;
+                            if (rr > sn) break;
+                            Row4 w = cell_row(a, b, f4, k, e, first, last, br1, br2, skind, ekind, sn, ub1, ub2, TZ_l,
+                                              TZ_r);
+                            int ix = rr - rb;
+                            sA[ix] = w.A; sB[ix] = w.B; sC[ix] = w.C; sD[ix] = w.D;
+                        }
+                    }
+                }
+                __syncthreads();
+                int g0 = 0;
+                if (c == 0) {
+                    if (sn == 0) {
+                        xs = sD[0] / sB[0];
+                    } else {
+                        ap = -sC[0] / sB[0];
+                        bp = sD[0] / sB[0];
+                        if (lane == 0) { sAp[0] = ap; sBp[0] = bp; }
+                    }
+                    g0 = 1;
+                }
+                const int g1 = (re < sn - 1 ? re : sn - 1) - rb;
+                spec_forward(sA, sB, sC, sD, sAp, sBp, g0, g1, ap, bp);
+                if (re == sn && sn > 0) {
+                    double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
+                    xs = (D - (A * bp)) / (B + (A * ap));
+                }
+                __syncthreads();
+                if (c + 1 < nch) { /* factors of this chunk (rows rb .. re, all < sn) to the line buffers */
+                    for (int i = lane; i <= re - rb; i += kWave) { gAp[rb + i] = sAp[i]; gBp[rb + i] = sBp[i]; }
+                }
+            }
+            /* back substitution, last chunk first; x[r] replaces A[r] */
+            double xc = xs;
+            for (int c = nch - 1; c >= 0; --c) {
+                const int rb = RW * c;
+                const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+                if (c + 1 < nch) {
+                    __syncthreads();
+                    for (int i = lane; i <= re - rb; i += kWave) { sAp[i] = gAp[rb + i]; sBp[i] = gBp[rb + i]; }
+                    __syncthreads();
+                } else if (lane == 0) {
+                    sA[sn - rb] = xs;
+                }
+                const int hi = (re < sn - 1 ? re : sn - 1) - rb;
+                xc = spec_backward(sAp, sBp, sA, hi, xc);
+                __syncthreads();
+                /* scatter the unknowns of positions first .. last-1 */
+                for (int i = lane; i <= re - rb; i += kWave) {
+                    int r = rb + i;
+                    double x = sA[i];
+                    int k;
+                    bool isz;
+                    if (skind != S_OPEN) { k = first + (r >> 1); isz = (r & 1) == 0; }
+                    else { isz = (r & 1) == 1; k = first + ((r + 1) >> 1); }
+                    if (k < last) {
+                        if (isz) tz[at(k)] = x; else tq[at(k)] = x;
+                    }
+                    if (r == 0) sX[0] = x;
+                    if (r == sn) sX[1] = x;
+                    if (r == ru) sX[2] = x;
+                }
+            }
+        } else if (lane == 0) {
+            /* no system (one-cell run at an open start edge): the cell's
+             * preindex is still needed at the end */
+            Pre a = isU ? pre_uz(d, last, line) : pre_vz(d, line, last);
+            sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            if (first < last) {
+                if (br1) tq[at(first - 1)] = 0;
+                else if (dkQ1 && first == 2) tq[at(first - 1)] = ub1;
+                else if (sn >= 0) tq[at(first - 1)] = (sE[2] - TZ_l - sE[1] * sX[0]) / sE[0];
+            }
+            if (br2) {
+                tq[at(last)] = 0;
+                if (sn >= 0) tz[at(last)] = sX[1];
+            } else if (dkQ2 && last == dkfr) {
+                tq[at(last)] = ub2;
+                if (sn >= 0) tz[at(last)] = sX[1];
+            } else {
+                double prev = (first < last && sn >= 0) ? sX[2] : tq[at(last - 1)];
+                tq[at(last)] = (sE[5] - sE[3] * prev - TZ_r) / sE[4];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 /* Cell (i, j) of a thread of a per-row kernel: with `tiled` the blocks walk
  * the context's list of active 64-column row tiles (tiles that hold, or
  * border, a cell with khouot != 2: elsewhere the per-row kernels are at a
@@ -1826,6 +2008,12 @@ void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStr
 {
     if (isU) hipLaunchKernelGGL(k_adi_run<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
     else hipLaunchKernelGGL(k_adi_run<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
+}
+
+void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s)
+{
+    if (isU) hipLaunchKernelGGL(k_adi_spec<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
+    else hipLaunchKernelGGL(k_adi_spec<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
 void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s)

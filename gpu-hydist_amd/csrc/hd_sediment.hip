@@ -10,6 +10,7 @@
  * in the same operation order (contraction off).
  */
 #include "hd_internal.h"
+#include "hd_thomas.h"
 
 namespace {
 
@@ -497,6 +498,106 @@ __global__ void __launch_bounds__(64) k_fs_run(const HdDev* __restrict__ dv, dou
     }
 }
 
+/* k_fs_run with the speculative parallel Thomas of hd_thomas.h: rows of a
+ * run are assembled in chunks of 512 (eight per lane), the forward sweep and
+ * the back substitution run on all 64 lanes with exact verification, so the
+ * bits are those of the serial solve (tridiagSolver with jump 1,
+ * UVZSolver_multithread.cu:203-262, via the FS rows of Sediment_Transport.cu:
+ * 62-338). */
+template <bool isJ>
+__global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, double s_start, int ketdinh)
+{
+    const HdDevG& d = dev_view(dv);
+    constexpr int RW = 512;
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW]; /* rows; sA holds x after the forward sweep */
+    __shared__ double sAp[RW], sBp[RW];
+    __shared__ double sV[2]; /* clamped x of the first and last row */
+    const int N = d.N, M = d.M, W = d.W;
+    const int lane = threadIdx.x;
+    const int tw = 2 * (N > M ? N : M) + 4;
+    const int lmax = isJ ? M - 1 : N - 1;
+    const int cap = (isJ ? M + 3 : N + 3) * d.SL;
+    const int cnt = d.run_cnt[isJ ? 0 : 1] < cap ? d.run_cnt[isJ ? 0 : 1] : cap;
+    const int4* runs = isJ ? d.runs_u : d.runs_v;
+    const double t = d.tdev[0];
+    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+        const int4 rn = runs[ri];
+        const int line = rn.x, first = rn.y, last = rn.z;
+        if (line < 3 || line > lmax || last < first + 2) continue;
+        auto at = [&](int k) -> size_t { return isJ ? (size_t)k * W + line : (size_t)line * W + k; };
+        bool br1, br2;
+        wall_flags2(d, isJ, line, first, last, br1, br2);
+        const int sn = last - first - 2;
+        const int k0 = first + 1;
+        double* gAp = d.Ap + (size_t)line * tw + k0;
+        double* gBp = d.Bp + (size_t)line * tw + k0;
+        const int nch = sn / RW + 1;
+        double ap = 0, bp = 0, xs = 0;
+        for (int c = 0; c < nch; ++c) {
+            const int rb = RW * c;
+            const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+            for (int r = rb + lane; r <= re; r += 64) {
+                const int k = k0 + r;
+                const int i = isJ ? k : line, j = isJ ? line : k;
+                FsRow w = fs_row<isJ>(d, t, s_start, ketdinh, i, j, first, last, br1, br2);
+                sA[r - rb] = w.A; sB[r - rb] = w.B; sC[r - rb] = w.C; sD[r - rb] = w.D;
+            }
+            __syncthreads();
+            int g0 = 0;
+            if (c == 0) {
+                if (sn == 0) {
+                    xs = sD[0] / sB[0];
+                } else {
+                    ap = -sC[0] / sB[0];
+                    bp = sD[0] / sB[0];
+                    if (lane == 0) { sAp[0] = ap; sBp[0] = bp; }
+                }
+                g0 = 1;
+            }
+            const int g1 = ((re < sn - 1) ? re : sn - 1) - rb;
+            hd_spec::spec_forward(sA, sB, sC, sD, sAp, sBp, g0, g1, ap, bp);
+            if (re == sn && sn > 0) {
+                double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
+                xs = (D - (A * bp)) / (B + (A * ap));
+            }
+            __syncthreads();
+            if (c + 1 < nch)
+                for (int q = lane; q <= re - rb; q += 64) { gAp[rb + q] = sAp[q]; gBp[rb + q] = sBp[q]; }
+        }
+        double xc = xs;
+        for (int c = nch - 1; c >= 0; --c) {
+            const int rb = RW * c;
+            const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
+            if (c + 1 < nch) {
+                __syncthreads();
+                for (int q = lane; q <= re - rb; q += 64) { sAp[q] = gAp[rb + q]; sBp[q] = gBp[rb + q]; }
+                __syncthreads();
+            } else if (lane == 0) {
+                sA[sn - rb] = xs;
+            }
+            const int hi = ((re < sn - 1) ? re : sn - 1) - rb;
+            xc = hd_spec::spec_backward(sAp, sBp, sA, hi, xc);
+            __syncthreads();
+            for (int q = lane; q <= re - rb; q += 64) {
+                double v = sA[q];
+                if (v < 0) v = d.P.NDnen;
+                d.tFS[at(k0 + rb + q)] = v;
+                if (rb + q == 0) sV[0] = v;
+                if (rb + q == sn) sV[1] = v;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const double* tqq = isJ ? d.t_u : d.t_v;
+            if ((br1) || (tqq[at(first)] == 0)) d.tFS[at(first)] = sV[0];
+            else d.tFS[at(first)] = d.FS[at(first)];
+            if ((br2) || (tqq[at(last)] == 0)) d.tFS[at(last)] = sV[1];
+            else d.tFS[at(last)] = d.FS[at(last)];
+        }
+        __syncthreads();
+    }
+}
+
 /* Update_FS, Sediment_Transport.cu:620-629. */
 __global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv)
 {
@@ -665,8 +766,14 @@ void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh,
     else hipLaunchKernelGGL(k_fs_line<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
 }
 
-void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, hipStream_t s)
+void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, int spec, hipStream_t s)
 {
-    if (which == 0) hipLaunchKernelGGL(k_fs_run<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
-    else hipLaunchKernelGGL(k_fs_run<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, s_start, ketdinh);
+    dim3 g(nblocks, 1, b.n);
+    if (spec) {
+        if (which == 0) hipLaunchKernelGGL(k_fs_spec<true>, g, dim3(64), 0, s, b.dv, s_start, ketdinh);
+        else hipLaunchKernelGGL(k_fs_spec<false>, g, dim3(64), 0, s, b.dv, s_start, ketdinh);
+    } else {
+        if (which == 0) hipLaunchKernelGGL(k_fs_run<true>, g, dim3(64), 0, s, b.dv, s_start, ketdinh);
+        else hipLaunchKernelGGL(k_fs_run<false>, g, dim3(64), 0, s, b.dv, s_start, ketdinh);
+    }
 }

@@ -1,0 +1,256 @@
+/*
+ * hd_thomas.h -- the speculative parallel Thomas solve shared by the ADI and
+ * FS run kernels (one wave of 64 lanes per tridiagonal system, rows in LDS).
+ */
+#ifndef HD_THOMAS_H
+#define HD_THOMAS_H
+
+#include <hip/hip_runtime.h>
+
+namespace hd_spec {
+
+constexpr int kSpecWave = 64;
+
+/* ------------------------------------------- speculative parallel Thomas
+ *
+ * The forward elimination of tridiag (UVZSolver_multithread.cu:203-228),
+ *   e_i = A_i ap_{i-1} + B_i,  ap_i = -C_i / e_i,  bp_i = (D_i - A_i bp_{i-1}) / e_i,
+ * is a serial chain of a multiply, an add and a correctly rounded division per
+ * row.  In exact arithmetic (ap, bp) = (p / q, r / q) where (p, q, r) follows
+ * the linear recurrence v_i = M_i v_{i-1},
+ *   M_i = [[0, -C_i, 0], [A_i, B_i, 0], [0, D_i, -A_i]],
+ * so each lane composes the matrices of its own rows, an inclusive scan over
+ * the wave composes the prefixes, and every lane obtains an estimate, good to
+ * a few ulps, of the state after each earlier lane's rows.  The recurrence
+ * contracts on these systems (each continuity / momentum pair reduces to a
+ * diagonally dominant system in z), so two rounded trajectories that start a
+ * few ulps apart become bit-identical after a few tens of rows.  A lane
+ * therefore runs the exact recurrence -- the same IEEE operations in the same
+ * order as the serial solve -- from the estimate of the state kSpecWF rows
+ * (rounded up to whole lanes) before its own rows, through its own rows, and
+ * the result is verified exactly: the state a lane had on entering its rows
+ * must equal, bit for bit, the state its predecessor produced at its last
+ * row.  Lane 0 starts from the exact state; by induction every lane that
+ * passes is exact, and the first lane that fails is recomputed serially from
+ * its verified predecessor, after which its successor is tested again.  The
+ * result equals the serial solve's bits in every case; the speculation only
+ * shortens the critical path from the run's length to about kSpecWF + L rows
+ * (L = rows per lane).  Back substitution, x_i = bp_i + ap_i x_{i+1}, is an
+ * affine chain and is treated the same way with 2-entry affine maps. */
+constexpr int kSpecWF = 24;           /* forward warm-up rows after the estimate */
+constexpr int kSpecWB = 32;           /* back-substitution warm-up rows */
+constexpr int kSpecSerialF = 48;      /* up to this many rows the forward sweep stays on one lane */
+constexpr int kSpecSerialB = 96;      /* likewise for back substitution */
+
+struct Mx {
+    double a, b, c, d, e, f, g; /* [[a b 0] [c d 0] [e f g]] */
+};
+
+/* M_i X for the row matrix M_i of (A, B, C, D) */
+__device__ __forceinline__ Mx mx_row(double A, double B, double C, double D, const Mx& X)
+{
+    Mx R;
+    R.a = -C * X.c;
+    R.b = -C * X.d;
+    R.c = fma(A, X.a, B * X.c);
+    R.d = fma(A, X.b, B * X.d);
+    R.e = fma(D, X.c, -A * X.e);
+    R.f = fma(D, X.d, -A * X.f);
+    R.g = -A * X.g;
+    return R;
+}
+
+/* X Y */
+__device__ __forceinline__ Mx mx_mul(const Mx& X, const Mx& Y)
+{
+    Mx R;
+    R.a = fma(X.a, Y.a, X.b * Y.c);
+    R.b = fma(X.a, Y.b, X.b * Y.d);
+    R.c = fma(X.c, Y.a, X.d * Y.c);
+    R.d = fma(X.c, Y.b, X.d * Y.d);
+    R.e = fma(X.e, Y.a, fma(X.f, Y.c, X.g * Y.e));
+    R.f = fma(X.e, Y.b, fma(X.f, Y.d, X.g * Y.f));
+    R.g = X.g * Y.g;
+    return R;
+}
+
+/* Scale by a power of two (exact) so the largest entry lies in [0.5, 1):
+ * only ratios of the composed state matter. */
+__device__ __forceinline__ void mx_norm(Mx& X)
+{
+    double m = fmax(fmax(fmax(fabs(X.a), fabs(X.b)), fmax(fabs(X.c), fabs(X.d))),
+                    fmax(fmax(fabs(X.e), fabs(X.f)), fabs(X.g)));
+    if (m > 0 && m <= 1.7976931348623157e308) {
+        int ex;
+        frexp(m, &ex);
+        double s = ldexp(1.0, -ex);
+        X.a *= s; X.b *= s; X.c *= s; X.d *= s; X.e *= s; X.f *= s; X.g *= s;
+    }
+}
+
+__device__ __forceinline__ Mx mx_shfl_up(const Mx& X, int dd)
+{
+    Mx Y;
+    Y.a = __shfl_up(X.a, dd, kSpecWave); Y.b = __shfl_up(X.b, dd, kSpecWave);
+    Y.c = __shfl_up(X.c, dd, kSpecWave); Y.d = __shfl_up(X.d, dd, kSpecWave);
+    Y.e = __shfl_up(X.e, dd, kSpecWave); Y.f = __shfl_up(X.f, dd, kSpecWave);
+    Y.g = __shfl_up(X.g, dd, kSpecWave);
+    return Y;
+}
+
+__device__ __forceinline__ bool same_bits(double x, double y)
+{
+    return __double_as_longlong(x) == __double_as_longlong(y);
+}
+
+/* One exact forward row, in the operation order of tridiag. */
+__device__ __forceinline__ void thomas_row(const double* sA, const double* sB, const double* sC, const double* sD,
+                                           int i, double& ap, double& bp)
+{
+    double A = sA[i], B = sB[i], C = sC[i], D = sD[i];
+    double e = A * ap + B;
+    double nap = -C / e;
+    double nbp = (D - (A * bp)) / e;
+    ap = nap;
+    bp = nbp;
+}
+
+/* Forward elimination of LDS rows g0 .. g1 (chunk-relative), entering with
+ * the exact state (ap, bp) of row g0 - 1, which is left as the state of row
+ * g1; the factors of the rows go to sAp / sBp.  One wave, uniform control. */
+__device__ __forceinline__ void spec_forward(const double* sA, const double* sB, const double* sC, const double* sD,
+                                             double* sAp, double* sBp, int g0, int g1, double& ap, double& bp)
+{
+    const int lane = threadIdx.x;
+    const int n = g1 - g0 + 1;
+    if (n <= 0) return;
+    const int L = n <= kSpecSerialF ? n : (n + kSpecWave - 1) / kSpecWave;
+    const int sk = g0 + lane * L;
+    const int ek = sk + L <= g1 + 1 ? sk + L : g1 + 1;
+    const bool act = sk <= g1;
+    const int m = (kSpecWF + L - 1) / L;
+    const int j = lane - m - 1;
+    double ca = ap, cb = bp;
+    int i0 = g0;
+    if (n > kSpecSerialF) {
+        /* prefix estimates */
+        Mx X{1, 0, 0, 1, 0, 0, 1};
+        for (int i = sk; i < ek; ++i) {
+            X = mx_row(sA[i], sB[i], sC[i], sD[i], X);
+            if (((i - sk) & 3) == 3) mx_norm(X);
+        }
+        mx_norm(X);
+        for (int dd = 1; dd < kSpecWave; dd <<= 1) {
+            Mx Y = mx_shfl_up(X, dd);
+            if (lane >= dd) {
+                X = mx_mul(X, Y);
+                mx_norm(X);
+            }
+        }
+        double p = fma(X.a, ap, X.b), q = fma(X.c, ap, X.d), r = fma(X.e, ap, fma(X.g, bp, X.f));
+        double ea = p / q, eb = r / q;
+        int src = j < 0 ? 0 : j;
+        ea = __shfl(ea, src, kSpecWave);
+        eb = __shfl(eb, src, kSpecWave);
+        if (j >= 0) { ca = ea; cb = eb; i0 = g0 + (j + 1) * L; }
+    }
+    const bool exact = j < 0;
+    if (!act) i0 = ek;
+    double wa = ca, wb = cb;
+    for (int i = i0; i < ek; ++i) {
+        if (i == sk) { wa = ca; wb = cb; }
+        thomas_row(sA, sB, sC, sD, i, ca, cb);
+        if (i >= sk) { sAp[i] = ca; sBp[i] = cb; }
+    }
+    if (n > kSpecSerialF) {
+        double pa = __shfl_up(ca, 1, kSpecWave), pb = __shfl_up(cb, 1, kSpecWave);
+        bool ok = exact || !act || (same_bits(wa, pa) && same_bits(wb, pb));
+        unsigned long long bad = __ballot(!ok);
+        while (bad) {
+            const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
+            __syncthreads();
+            if (lane == f) {
+                ca = sAp[sk - 1];
+                cb = sBp[sk - 1];
+                for (int i = sk; i < ek; ++i) {
+                    thomas_row(sA, sB, sC, sD, i, ca, cb);
+                    sAp[i] = ca; sBp[i] = cb;
+                }
+            }
+            double na = __shfl(ca, f, kSpecWave), nb = __shfl(cb, f, kSpecWave);
+            if (lane == f) ok = true;
+            if (lane == f + 1) ok = exact || !act || (same_bits(wa, na) && same_bits(wb, nb));
+            bad = __ballot(!ok);
+        }
+    }
+    const int klast = (n - 1) / L;
+    ap = __shfl(ca, klast, kSpecWave);
+    bp = __shfl(cb, klast, kSpecWave);
+}
+
+/* Back substitution over LDS rows hi down to 0 (chunk-relative) given x of
+ * row hi + 1; x_i goes to sX[i].  Returns x of row 0 (x_in when hi < 0). */
+__device__ __forceinline__ double spec_backward(const double* sAp, const double* sBp, double* sX, int hi, double x_in)
+{
+    const int lane = threadIdx.x;
+    const int n = hi + 1;
+    if (n <= 0) return x_in;
+    const int L = n <= kSpecSerialB ? n : (n + kSpecWave - 1) / kSpecWave;
+    const int top = hi - lane * L;
+    const int bot = top - L + 1 >= 0 ? top - L + 1 : 0;
+    const bool act = top >= 0;
+    const int m = (kSpecWB + L - 1) / L;
+    const int j = lane - m - 1;
+    double cx = x_in;
+    int i0 = hi;
+    if (n > kSpecSerialB) {
+        double al = 1, be = 0;
+        for (int i = top; i >= bot; --i) {
+            be = fma(sAp[i], be, sBp[i]);
+            al = sAp[i] * al;
+        }
+        for (int dd = 1; dd < kSpecWave; dd <<= 1) {
+            double a2 = __shfl_up(al, dd, kSpecWave), b2 = __shfl_up(be, dd, kSpecWave);
+            if (lane >= dd) {
+                be = fma(al, b2, be);
+                al = al * a2;
+            }
+        }
+        double est = fma(al, x_in, be);
+        est = __shfl(est, j < 0 ? 0 : j, kSpecWave);
+        if (j >= 0) { cx = est; i0 = hi - (j + 1) * L; }
+    }
+    const bool exact = j < 0;
+    if (!act) i0 = bot - 1;
+    double w = cx;
+    for (int i = i0; i >= bot; --i) {
+        if (i == top) w = cx;
+        cx = sBp[i] + (sAp[i] * cx);
+        if (i <= top) sX[i] = cx;
+    }
+    if (n > kSpecSerialB) {
+        double px = __shfl_up(cx, 1, kSpecWave);
+        bool ok = exact || !act || same_bits(w, px);
+        unsigned long long bad = __ballot(!ok);
+        while (bad) {
+            const int f = __ffsll((long long)bad) - 1;
+            __syncthreads();
+            if (lane == f) {
+                cx = sX[top + 1];
+                for (int i = top; i >= bot; --i) {
+                    cx = sBp[i] + (sAp[i] * cx);
+                    sX[i] = cx;
+                }
+            }
+            double nx = __shfl(cx, f, kSpecWave);
+            if (lane == f) ok = true;
+            if (lane == f + 1) ok = exact || !act || same_bits(w, nx);
+            bad = __ballot(!ok);
+        }
+    }
+    return __shfl(cx, (n - 1) / L, kSpecWave);
+}
+
+} // namespace hd_spec
+
+#endif

@@ -276,7 +276,11 @@ def river_case(N: int = 980, M: int = 1048, dx: float = 20.0, dt: float = 5.0, h
         series = {"bc_down": qs, "bc_up": zs}
         btype = (1, 2, 0, 0)
         cc_key = "CC_d"
-    over = dict(H_TINH=H_TINH, heso=0.94)
+    # heso: weight of a face velocity against its neighbour mean in the
+    # smoothing pass (Normalize, support_funcs.cu:682-703); at the channel
+    # case's 0.94 this river's run fragments its wet mask within hours, at
+    # 0.8 it integrates stably for days
+    over = dict(H_TINH=H_TINH, heso=0.8)
     kw = {}
     if sediment:
         over.update(dm=2e-4, ros=2650.0, Dorong=0.4, Sx=2.65, NDnen=0.0, Toe=1.0)

@@ -171,10 +171,22 @@ def test_full_river_bitwise():
     assert_same_state(o, g, "river 980x1048, 4 steps")
 
 
-@pytest.mark.parametrize("form,make", [("line", small_river), ("rows", small_river),
-                                       ("line", small_river_sediment), ("rows", small_river_sediment)])
-def test_solver_forms_identical(form, make, monkeypatch):
-    """The three solver forms (one wave per run, one wave per line, rows
+def full_river():
+    return synth.config(2)
+
+
+def long_channel():
+    # 600 cells along the UZ lines: runs of 1200 rows span three LDS chunks
+    return synth.channel_case(N=600, M=60, nsteps=200)
+
+
+@pytest.mark.parametrize("form,make,steps", [("line", small_river, 30), ("rows", small_river, 30),
+                                             ("line", small_river_sediment, 30), ("rows", small_river_sediment, 30),
+                                             ("run", small_river, 30), ("run", full_river, 60),
+                                             ("run", long_channel, 60)])
+def test_solver_forms_identical(form, make, steps, monkeypatch):
+    """The solver forms (one wave per run with the speculative parallel
+    Thomas, one wave per run with the serial Thomas, one wave per line, rows
     kernel + lane per line), hydrodynamics and sediment, give identical bits."""
     case = make()
     ref = Solver(case)
@@ -184,9 +196,9 @@ def test_solver_forms_identical(form, make, monkeypatch):
     ref.init()
     alt.init()
     op = case.opts()
-    ref.run(0.0, 30, op)
-    alt.run(0.0, 30, op)
-    assert_same_state(ref, alt, f"run-packed vs {form}")
+    ref.run(0.0, steps, op)
+    alt.run(0.0, steps, op)
+    assert_same_state(ref, alt, f"speculative vs {form}")
 
 
 def test_clone_copies_state_and_steps_identically():
