@@ -43,6 +43,7 @@ KERNEL_BYTES = {
     # (mirror for u), t_v (t_u) written
     "explicit_margins_v": 9 * 8 + 8,
     "explicit_margins_u": 9 * 8 + 8,
+    "explicit_rows_u": 9 * 8 + 8,
     # state copy + drying + face depths: t_z, t_u, t_v, H_moi, h read;
     # z, u, v, Htdu, Htdv written
     "uvz_dry_htd": 5 * 8 + 5 * 8,

@@ -345,6 +345,13 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
         size_t ncb = (size_t)(M + 3 + 63) / 64;
         if ((rc = dalloc(c, (void**)&d.tiles, (size_t)(N + 3) * ncb * 4))) { hd_destroy(c); return rc; }
         if ((rc = dalloc(c, (void**)&d.ntiles, 4))) { hd_destroy(c); return rc; }
+        /* line extents: the whole line until the mask is known */
+        std::vector<int2> ext((size_t)(N + 3) + (M + 3), make_int2(0, 1 << 30));
+        if ((rc = dalloc(c, (void**)&d.ext, ext.size() * sizeof(int2)))) { hd_destroy(c); return rc; }
+        if (hipMemcpy(d.ext, ext.data(), ext.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+            hd_destroy(c);
+            return fail(nullptr, HD_EHIP, "hipMemcpy failed");
+        }
     }
     c->t_u0 = d.t_u;
     c->t_v0 = d.t_v;
@@ -429,6 +436,8 @@ static int check_flags(hd_ctx* c)
     return HD_OK;
 }
 
+static int compute_tiles(hd_ctx* c);
+
 int hd_upload(hd_ctx* c, const char* name, const void* src, size_t nbytes)
 {
     void* p;
@@ -441,6 +450,8 @@ int hd_upload(hd_ctx* c, const char* name, const void* src, size_t nbytes)
     HIPCHK(c, hipMemcpyAsync(p, src, nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->settled = false;
+    /* a new mask can open cells outside the active tiles */
+    if (c->initialised && std::strcmp(name, "khouot") == 0) return compute_tiles(c);
     return HD_OK;
 }
 
@@ -493,6 +504,18 @@ static int compute_tiles(hd_ctx* c)
     std::vector<int> list;
     for (size_t t = 0; t < live.size(); ++t)
         if (live[t]) list.push_back((int)t);
+    /* per-line extents of the cells that can ever be wet (khouot != 2 stays
+     * so: only khouot == 1 cells reopen) */
+    std::vector<int2> ext((size_t)(N + 3) + (M + 3), make_int2(1, 0));
+    for (int i = 0; i < N + 3; ++i)
+        for (int j = 0; j < M + 3; ++j)
+            if (kh[(size_t)i * W + j] != 2) {
+                int2& r = ext[i];
+                if (r.x > r.y) r = make_int2(j, j); else r.y = j;
+                int2& q = ext[(size_t)(N + 3) + j];
+                if (q.x > q.y) q = make_int2(i, i); else { q.x = std::min(q.x, i); q.y = std::max(q.y, i); }
+            }
+    HIPCHK(c, hipMemcpyAsync(c->d.ext, ext.data(), ext.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     int n = (int)list.size();
     if (n) HIPCHK(c, hipMemcpyAsync(c->d.tiles, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d.ntiles, &n, 4, hipMemcpyHostToDevice, c->stream));
@@ -590,7 +613,8 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
         x.swap_u();
         if (x.solver == 0 || x.solver == 3) {
-            launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, s); });
+            /* rows are coalesced already: one wave per line, margins after the cells */
+            launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, 1, s); });
         } else {
             launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
             launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
@@ -613,8 +637,12 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         }
         launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
         x.swap_v();
-        if (x.solver == 0 || x.solver == 3) {
-            launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, s); });
+        if (x.solver == 0) {
+            /* columns: the cells by row tiles (coalesced), then the margins per line */
+            launch(x, "explicit_rows_u", [&] { hd_launch_explicit_rows_u(x.b(), x.bw, ntiles, s); });
+            launch(x, "margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 0, s); });
+        } else if (x.solver == 3) {
+            launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 1, s); });
         } else {
             launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
             launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });

@@ -35,8 +35,11 @@
  * runs_u / runs_v packed wet runs (line, first, last, 0) of the UZ / VZ
  * sweeps and run_cnt their entry counts; flags [0] segment overflow, [1]
  * boundary range; wet_count [0] run accumulator, [1] one-shot count; tdev
- * the device time; tiles / ntiles the active 64-column row tiles; cand_cnt /
- * cand the wetting candidates (64 per line); dryflag one byte per cell. */
+ * the device time; tiles / ntiles the active 64-column row tiles; ext the
+ * first and last position of each row (0 .. N+2) and then each column
+ * (0 .. M+2) holding a cell with khouot != 2 (outside it no cell is ever
+ * wet); cand_cnt / cand the wetting candidates (64 per line); dryflag one
+ * byte per cell. */
 #define HD_DEV_MEMBERS(G) \
     int N, M, W, SL, total_time, series_len; \
     hd_params P; \
@@ -58,6 +61,7 @@
     G double* tdev; \
     G int* tiles; \
     G int* ntiles; \
+    G int2* ext; \
     G int *cand_cnt, *cand; \
     G unsigned char* dryflag;
 
@@ -107,7 +111,8 @@ void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, hipSt
 void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, int ntiles, hipStream_t s);
 void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
-void hd_launch_explicit_margins(const HdBatch& b, int isU, hipStream_t s);
+void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_t s);
+void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
 void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);

@@ -138,53 +138,6 @@ __global__ void k_preprocess(const HdDev* __restrict__ dv)
 
 /* ================================================================ boundary */
 
-/* Boundary_value + FS_boundary for thread index i of one edge,
- * support_funcs.cu:544-617.  Only segment 0 of the edge line can match
- * (the `break` at support_funcs.cu:558). */
-__device__ void edge_value(const HdDevG& d, int edge, int i, int t1, double t2)
-{
-    int N = d.N, M = d.M, W = d.W, SL = d.SL, TT = d.total_time;
-    const int* bt = d.boundary_type + edge * SL;
-    const int *moc, *dau, *cuoi;
-    int location, loc_ext;
-    bool isU;
-    const double *bc, *CCs, *hi;
-    double* barr;
-    double hmax = d.hmax[edge];
-    if (edge == 0) { isU = false; location = M; loc_ext = M + 1; moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; bc = d.bc_up; CCs = d.CC_u; hi = d.hi; barr = d.vbt; }
-    else if (edge == 1) { isU = false; location = 2; loc_ext = 1; moc = d.mocj; dau = d.dauj; cuoi = d.cuoij; bc = d.bc_down; CCs = d.CC_d; hi = d.hi + (N + 3); barr = d.vbd; }
-    else if (edge == 2) { isU = true; location = 2; loc_ext = 1; moc = d.moci; dau = d.daui; cuoi = d.cuoii; bc = d.bc_left; CCs = d.CC_l; hi = d.hi + 2 * (N + 3); barr = d.ubt; }
-    else { isU = true; location = N; loc_ext = N + 1; moc = d.moci; dau = d.daui; cuoi = d.cuoii; bc = d.bc_right; CCs = d.CC_r; hi = d.hi + 2 * (N + 3) + (M + 3); barr = d.ubp; }
-    if (!(moc[location] > 0 && dau[location * SL] <= i && i <= cuoi[location * SL])) return;
-    const int seg_no = 0;
-    /* a column edge (0, 1) leaves its cell of row 2 / N to the row edge
-     * (2 / 3) that writes the same field there: in the reference's edge
-     * order the later edge wins */
-    bool yield = false;
-    if (!isU && (i == 2 || i == N)) {
-        int re = (i == 2) ? 2 : 3;
-        const int* rbt = d.boundary_type + re * SL;
-        int rloc = (i == 2) ? 2 : N;
-        bool covers = d.moci[rloc] > 0 && d.daui[rloc * SL] <= location && location <= d.cuoii[rloc * SL];
-        yield = covers && ((rbt[0] == 2) == (bt[seg_no] == 2));
-    }
-    double bv = bc[seg_no * TT + t1] * (1.0 - t2) + bc[seg_no * TT + t1 + 1] * t2;
-    if (bt[seg_no] == 2) {
-        barr[i] = bv * hi[i];
-    } else if (isU) {
-        d.t_z[location * W + i] = bv;
-        d.t_z[loc_ext * W + i] = bv;
-    } else {
-        if (!yield) d.t_z[i * W + location] = bv;
-        d.t_z[i * W + loc_ext] = bv;
-    }
-    if (bt[seg_no] != 2 || yield) return;
-    int fi, fj;
-    if (isU) { fi = location; fj = i; } else { fj = location; fi = i; }
-    double cv = CCs[seg_no * TT + t1] * (1.0 - t2) + CCs[seg_no * TT + t1 + 1] * t2;
-    d.FS[fi * W + fj] = cv * d.htaiz_bd[fi * W + fj] * hmax;
-}
-
 /* boundary_up/down/left/right, support_funcs.cu:356-429 (analytic tide of the
  * narrow-channel mode), thread index i of one edge. */
 __device__ void edge_analytic(const HdDevG& d, int edge, int i, double t)
@@ -222,18 +175,19 @@ __device__ void edge_analytic(const HdDevG& d, int edge, int i, double t)
 /* Update_Boundary_Value, support_funcs.cu:622-668 (or the analytic channel
  * boundaries), one workgroup; the edges run in the reference's order
  * up, down, left, right, so a corner cell keeps the later edge's value.
- * The half-step time t = t_prev + dT/2 is advanced here, on the device. */
+ * The half-step time t = t_prev + dT/2 is advanced here, on the device.
+ * Every thread reads the device time and its edge's tables in one round of
+ * loads; the series values and the cells' weights follow in a second. */
 __global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
 {
     const HdDevG& d = dev_view(dv);
     double* tdev = d.tdev;
-    __shared__ double t_s;
-    if (threadIdx.x == 0) t_s = tdev[0] + d.P.dT * 0.5;
-    __syncthreads();
-    double t = t_s;
-    if (threadIdx.x == 0) tdev[0] = t;
-    int imax = d.M > d.N ? d.M : d.N;
+    const double t = tdev[0] + d.P.dT * 0.5;
+    const int N = d.N, M = d.M, W = d.W, SL = d.SL, TT = d.total_time;
+    const int imax = M > N ? M : N;
     if (analytic) {
+        __syncthreads(); /* every thread has read the old time */
+        if (threadIdx.x == 0) tdev[0] = t;
         int lo = 0, hi = imax + 2;
         for (int e = 0; e < 4; ++e) {
             for (int i = lo + threadIdx.x; i <= hi; i += blockDim.x) edge_analytic(d, e, i, t);
@@ -241,24 +195,64 @@ __global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
         }
         return;
     }
-    int t1 = (int)(t / 3600);
-    double t2 = (t - (3600.0 * t1)) / 3600.0;
-    if (t1 < 0 || t1 + 1 >= d.total_time || t1 + 1 >= d.series_len) {
+    /* the four edges at once, 256 threads each, over the edge line's first
+     * wet run (the only one the reference updates, support_funcs.cu:558) */
+    const int e = threadIdx.x >> 8, lt = threadIdx.x & 255;
+    const bool isU = e >= 2;
+    const int* moc = isU ? d.moci : d.mocj;
+    const int* dau = isU ? d.daui : d.dauj;
+    const int* cuoi = isU ? d.cuoii : d.cuoij;
+    const int location = e == 0 ? M : (e == 3 ? N : 2);
+    const int loc_ext = e == 0 ? M + 1 : (e == 3 ? N + 1 : 1);
+    const int mc = moc[location], a = dau[location * SL], b = cuoi[location * SL];
+    const int bt0 = d.boundary_type[e * SL];
+    const double hmax = d.hmax[e];
+    /* row-edge coverage of the corner cells (rows 2 and N) of the column edges */
+    const int m2 = d.moci[2], a2 = d.daui[2 * SL], b2 = d.cuoii[2 * SL], rbt2 = d.boundary_type[2 * SL];
+    const int mN = d.moci[N], aN = d.daui[N * SL], bN = d.cuoii[N * SL], rbt3 = d.boundary_type[3 * SL];
+    __syncthreads(); /* every thread has read the old time */
+    if (threadIdx.x == 0) tdev[0] = t;
+    const int t1 = (int)(t / 3600);
+    const double t2 = (t - (3600.0 * t1)) / 3600.0;
+    if (t1 < 0 || t1 + 1 >= TT || t1 + 1 >= d.series_len) {
         if (threadIdx.x == 0) d.flags[1] = 1;
         return;
     }
-    /* the four edges at once, 256 threads each, over the edge line's first
-     * wet run (the only one the reference updates); corner cells go to the
-     * later edge by the rule in edge_value */
-    const int e = threadIdx.x >> 8, lt = threadIdx.x & 255;
-    const int SL = d.SL;
-    const int* moc = e < 2 ? d.mocj : d.moci;
-    const int* dau = e < 2 ? d.dauj : d.daui;
-    const int* cuoi = e < 2 ? d.cuoij : d.cuoii;
-    const int location = e == 0 ? d.M : (e == 3 ? d.N : 2);
-    if (moc[location] <= 0) return;
-    const int a = dau[location * SL], b = cuoi[location * SL];
-    for (int i = (a > 2 ? a : 2) + lt; i <= b && i <= imax; i += 256) edge_value(d, e, i, t1, t2);
+    if (mc <= 0) return;
+    const double* bc = e == 0 ? d.bc_up : e == 1 ? d.bc_down : e == 2 ? d.bc_left : d.bc_right;
+    const double* CCs = e == 0 ? d.CC_u : e == 1 ? d.CC_d : e == 2 ? d.CC_l : d.CC_r;
+    const double* hi = d.hi + (e == 0 ? 0 : e == 1 ? (N + 3) : e == 2 ? 2 * (N + 3) : 2 * (N + 3) + (M + 3));
+    double* barr = e == 0 ? d.vbt : e == 1 ? d.vbd : e == 2 ? d.ubt : d.ubp;
+    for (int i = (a > 2 ? a : 2) + lt; i <= b && i <= imax; i += 256) {
+        /* Boundary_value + FS_boundary, support_funcs.cu:544-617, segment 0 */
+        const int fi = isU ? location : i, fj = isU ? i : location;
+        const double b0 = bc[t1], b1 = bc[t1 + 1];
+        const double c0 = CCs[t1], c1 = CCs[t1 + 1];
+        const double hw = hi[i], hbd = d.htaiz_bd[fi * W + fj];
+        /* a column edge (0, 1) leaves its cell of row 2 / N to the row edge
+         * (2 / 3) that writes the same field there: in the reference's edge
+         * order the later edge wins */
+        bool yield = false;
+        if (!isU && (i == 2 || i == N)) {
+            const bool covers = (i == 2) ? (m2 > 0 && a2 <= location && location <= b2)
+                                         : (mN > 0 && aN <= location && location <= bN);
+            const int rb0 = (i == 2) ? rbt2 : rbt3;
+            yield = covers && ((rb0 == 2) == (bt0 == 2));
+        }
+        const double bv = b0 * (1.0 - t2) + b1 * t2;
+        if (bt0 == 2) {
+            barr[i] = bv * hw;
+        } else if (isU) {
+            d.t_z[location * W + i] = bv;
+            d.t_z[loc_ext * W + i] = bv;
+        } else {
+            if (!yield) d.t_z[i * W + location] = bv;
+            d.t_z[i * W + loc_ext] = bv;
+        }
+        if (bt0 != 2 || yield) continue;
+        const double cv = c0 * (1.0 - t2) + c1 * t2;
+        d.FS[fi * W + fj] = cv * hbd * hmax;
+    }
 }
 
 /* ================================================================ ADI rows */
@@ -267,81 +261,116 @@ struct Pre {
     double f1, f2, f3, f5, a2, c2, d2;
 };
 
-/* _uzSolver_calculate_preindex, UVZSolver_multithread.cu:352-407, cell (i, j). */
-__device__ Pre pre_uz(const HdDevG& d, int i, int j)
+/* Every value a preindex cell may read, loaded unconditionally so that all
+ * loads of a cell (20, all in bounds for 2 <= i <= N, 2 <= j <= M) are in
+ * flight together; the evaluation below takes the same branches and performs
+ * the same operations as the reference on them. */
+struct PreRaw {
+    double c, n1, n2, m1, m2;   /* along-direction velocity at the cell and its +-1, +-2 cross neighbours */
+    double x0, x1, x2, x3;      /* the four cross velocities averaged at the face */
+    double vis, ht, ht_m, k1;   /* VISCOIDX, face depth here and one position back, Kx1 | Ky1 */
+    double hm_m1, hm_p1, hm_m2; /* H_moi at -1, +1, -2 */
+    double ts, zz, hc0, hc1;    /* Tsxw | Tsyw, z, cross face depths */
+};
+
+/* loads of _uzSolver_calculate_preindex (UVZSolver_multithread.cu:352-407) at (i, j) */
+__device__ __forceinline__ PreRaw pre_uz_ld(const HdDevG& d, int i, int j)
 {
-    const hd_params& P = d.P;
-    int W = d.W;
-    const double *u = d.u, *v = d.v, *Hm = d.H_moi, *VIS = d.VISCOIDX;
-    Pre r;
-    double p = 0.0, q = 0.0;
-    double uc = u[i * W + j];
-    double vtb = (v[i * W + j - 1] + v[i * W + j] + v[(i + 1) * W + j - 1] + v[(i + 1) * W + j]) * 0.25;
-    double vis = VIS[i * W + j];
-    double htu = d.Htdu[i * W + j];
-    r.f1 = P.dTchia2dX * uc + vis * P.dT / P.dXbp;
-    r.f2 = -(2 + d.Kx1[i * W + j] * P.dT * sqrt(uc * uc + vtb * vtb) / htu + (2 * P.dT * vis) / P.dXbp);
-    r.f3 = P.dT * vis / P.dXbp - P.dTchia2dX * uc;
-    if (Hm[i * W + j - 1] <= P.H_TINH) {
-        if (vtb < 0) {
-            p = vtb * (-3 * uc + 4 * u[i * W + j + 1] - u[i * W + j + 2]) / P.dY2;
-            q = (uc - 2 * u[i * W + j + 1] + u[i * W + j + 2]) / P.dYbp;
-        }
-    } else {
-        if (Hm[i * W + j + 1] <= P.H_TINH) {
-            if ((Hm[i * W + j - 2] > P.H_TINH) && (vtb > 0)) {
-                p = vtb * (3 * uc - 4 * u[i * W + j - 1] + u[i * W + j - 2]) / P.dY2;
-                q = (uc - 2 * u[i * W + j - 1] + u[i * W + j - 2]) / P.dYbp;
-            }
-        } else {
-            p = vtb * (u[i * W + j + 1] - u[i * W + j - 1]) / P.dY2;
-            q = (u[i * W + j + 1] - 2 * uc + u[i * W + j - 1]) / P.dYbp;
-        }
-    }
-    r.f5 = -2 * uc + P.dT * p - P.dT * P.CORIOLIS_FORCE * vtb - P.dT * vis * q - P.dT * (P.Windx - d.Tsxw[i * W + j]) / htu;
-    r.c2 = P.dTchia2dX * htu;
-    r.a2 = -P.dTchia2dX * d.Htdu[(i - 1) * W + j];
-    r.d2 = d.z[i * W + j] - P.dTchia2dY * (d.Htdv[i * W + j] * v[i * W + j] - d.Htdv[i * W + j - 1] * v[i * W + j - 1]);
+    const int W = d.W, p = i * W + j;
+    PreRaw r;
+    r.c = d.u[p]; r.n1 = d.u[p + 1]; r.n2 = d.u[p + 2]; r.m1 = d.u[p - 1]; r.m2 = d.u[p - 2];
+    r.x0 = d.v[p - 1]; r.x1 = d.v[p]; r.x2 = d.v[p + W - 1]; r.x3 = d.v[p + W];
+    r.vis = d.VISCOIDX[p]; r.ht = d.Htdu[p]; r.ht_m = d.Htdu[p - W]; r.k1 = d.Kx1[p];
+    r.hm_m1 = d.H_moi[p - 1]; r.hm_p1 = d.H_moi[p + 1]; r.hm_m2 = d.H_moi[p - 2];
+    r.ts = d.Tsxw[p]; r.zz = d.z[p]; r.hc0 = d.Htdv[p]; r.hc1 = d.Htdv[p - 1];
     return r;
 }
 
-/* _vzSolver_calculate_preindex, UVZSolver_multithread.cu:295-350, cell (i, j). */
-__device__ Pre pre_vz(const HdDevG& d, int i, int j)
+/* _uzSolver_calculate_preindex, UVZSolver_multithread.cu:352-407, on the
+ * loaded values of cell (i, j). */
+__device__ __forceinline__ Pre pre_uz_ev(const hd_params& P, const PreRaw& r)
 {
-    const hd_params& P = d.P;
-    int W = d.W;
-    const double *u = d.u, *v = d.v, *Hm = d.H_moi, *VIS = d.VISCOIDX;
-    Pre r;
+    Pre o;
     double p = 0.0, q = 0.0;
-    double vc = v[i * W + j];
-    double utb = (u[(i - 1) * W + j] + u[i * W + j] + u[(i - 1) * W + j + 1] + u[i * W + j + 1]) * 0.25;
-    double vis = VIS[i * W + j];
-    double htv = d.Htdv[i * W + j];
-    r.f1 = P.dTchia2dY * vc + vis * P.dT / P.dYbp;
-    r.f2 = -(2 + d.Ky1[i * W + j] * P.dT * sqrt(vc * vc + utb * utb) / htv + (2 * P.dT * vis) / P.dYbp);
-    r.f3 = P.dT * vis / P.dYbp - P.dTchia2dY * vc;
-    if (Hm[(i - 1) * W + j] <= P.H_TINH) {
-        if (utb < 0) {
-            q = utb * (-3 * vc + 4 * v[(i + 1) * W + j] - v[(i + 2) * W + j]) / P.dX2;
-            p = (vc - 2 * v[(i + 1) * W + j] + v[(i + 2) * W + j]) / P.dXbp;
+    const double uc = r.c;
+    const double vtb = (r.x0 + r.x1 + r.x2 + r.x3) * 0.25;
+    const double vis = r.vis, htu = r.ht;
+    o.f1 = P.dTchia2dX * uc + vis * P.dT / P.dXbp;
+    o.f2 = -(2 + r.k1 * P.dT * sqrt(uc * uc + vtb * vtb) / htu + (2 * P.dT * vis) / P.dXbp);
+    o.f3 = P.dT * vis / P.dXbp - P.dTchia2dX * uc;
+    if (r.hm_m1 <= P.H_TINH) {
+        if (vtb < 0) {
+            p = vtb * (-3 * uc + 4 * r.n1 - r.n2) / P.dY2;
+            q = (uc - 2 * r.n1 + r.n2) / P.dYbp;
         }
     } else {
-        if (Hm[(i + 1) * W + j] <= P.H_TINH) {
-            if ((Hm[(i - 2) * W + j] > P.H_TINH) && (utb > 0)) {
-                q = utb * (3 * vc - 4 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / P.dX2;
-                p = (vc - 2 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / P.dXbp;
+        if (r.hm_p1 <= P.H_TINH) {
+            if ((r.hm_m2 > P.H_TINH) && (vtb > 0)) {
+                p = vtb * (3 * uc - 4 * r.m1 + r.m2) / P.dY2;
+                q = (uc - 2 * r.m1 + r.m2) / P.dYbp;
             }
         } else {
-            q = utb * (v[(i + 1) * W + j] - v[(i - 1) * W + j]) / P.dX2;
-            p = (v[(i + 1) * W + j] - 2 * vc + v[(i - 1) * W + j]) / P.dXbp;
+            p = vtb * (r.n1 - r.m1) / P.dY2;
+            q = (r.n1 - 2 * uc + r.m1) / P.dYbp;
         }
     }
-    r.f5 = -2 * vc + P.dT * q + P.dT * P.CORIOLIS_FORCE * utb - P.dT * vis * p - P.dT * (P.Windy - d.Tsyw[i * W + j]) / htv;
-    r.c2 = P.dTchia2dY * htv;
-    r.a2 = -P.dTchia2dY * d.Htdv[i * W + j - 1];
-    r.d2 = d.z[i * W + j] - P.dTchia2dX * (d.Htdu[i * W + j] * u[i * W + j] - d.Htdu[(i - 1) * W + j] * u[(i - 1) * W + j]);
+    o.f5 = -2 * uc + P.dT * p - P.dT * P.CORIOLIS_FORCE * vtb - P.dT * vis * q - P.dT * (P.Windx - r.ts) / htu;
+    o.c2 = P.dTchia2dX * htu;
+    o.a2 = -P.dTchia2dX * r.ht_m;
+    o.d2 = r.zz - P.dTchia2dY * (r.hc0 * r.x1 - r.hc1 * r.x0);
+    return o;
+}
+
+/* loads of _vzSolver_calculate_preindex (UVZSolver_multithread.cu:295-350) at (i, j) */
+__device__ __forceinline__ PreRaw pre_vz_ld(const HdDevG& d, int i, int j)
+{
+    const int W = d.W, p = i * W + j;
+    PreRaw r;
+    r.c = d.v[p]; r.n1 = d.v[p + W]; r.n2 = d.v[p + 2 * W]; r.m1 = d.v[p - W]; r.m2 = d.v[p - 2 * W];
+    r.x0 = d.u[p - W]; r.x1 = d.u[p]; r.x2 = d.u[p - W + 1]; r.x3 = d.u[p + 1];
+    r.vis = d.VISCOIDX[p]; r.ht = d.Htdv[p]; r.ht_m = d.Htdv[p - 1]; r.k1 = d.Ky1[p];
+    r.hm_m1 = d.H_moi[p - W]; r.hm_p1 = d.H_moi[p + W]; r.hm_m2 = d.H_moi[p - 2 * W];
+    r.ts = d.Tsyw[p]; r.zz = d.z[p]; r.hc0 = d.Htdu[p]; r.hc1 = d.Htdu[p - W];
     return r;
 }
+
+/* _vzSolver_calculate_preindex, UVZSolver_multithread.cu:295-350, on the
+ * loaded values of cell (i, j). */
+__device__ __forceinline__ Pre pre_vz_ev(const hd_params& P, const PreRaw& r)
+{
+    Pre o;
+    double p = 0.0, q = 0.0;
+    const double vc = r.c;
+    const double utb = (r.x0 + r.x1 + r.x2 + r.x3) * 0.25;
+    const double vis = r.vis, htv = r.ht;
+    o.f1 = P.dTchia2dY * vc + vis * P.dT / P.dYbp;
+    o.f2 = -(2 + r.k1 * P.dT * sqrt(vc * vc + utb * utb) / htv + (2 * P.dT * vis) / P.dYbp);
+    o.f3 = P.dT * vis / P.dYbp - P.dTchia2dY * vc;
+    if (r.hm_m1 <= P.H_TINH) {
+        if (utb < 0) {
+            q = utb * (-3 * vc + 4 * r.n1 - r.n2) / P.dX2;
+            p = (vc - 2 * r.n1 + r.n2) / P.dXbp;
+        }
+    } else {
+        if (r.hm_p1 <= P.H_TINH) {
+            if ((r.hm_m2 > P.H_TINH) && (utb > 0)) {
+                q = utb * (3 * vc - 4 * r.m1 + r.m2) / P.dX2;
+                p = (vc - 2 * r.m1 + r.m2) / P.dXbp;
+            }
+        } else {
+            q = utb * (r.n1 - r.m1) / P.dX2;
+            p = (r.n1 - 2 * vc + r.m1) / P.dXbp;
+        }
+    }
+    o.f5 = -2 * vc + P.dT * q + P.dT * P.CORIOLIS_FORCE * utb - P.dT * vis * p - P.dT * (P.Windy - r.ts) / htv;
+    o.c2 = P.dTchia2dY * htv;
+    o.a2 = -P.dTchia2dY * r.ht_m;
+    o.d2 = r.zz - P.dTchia2dX * (r.hc0 * r.x1 - r.hc1 * r.x0);
+    return o;
+}
+
+__device__ Pre pre_uz(const HdDevG& d, int i, int j) { return pre_uz_ev(d.P, pre_uz_ld(d, i, j)); }
+__device__ Pre pre_vz(const HdDevG& d, int i, int j) { return pre_vz_ev(d.P, pre_vz_ld(d, i, j)); }
 
 enum { S_WALL = 0, S_Q = 1, S_OPEN = 2 };
 enum { E_Q = 1, E_OPEN = 2, E_WALL = 3 };
@@ -1058,23 +1087,26 @@ __device__ __forceinline__ Pre shfl_pre(const Pre& a, int src)
 }
 
 /* Fused UZ / VZ solve of one wet run per wave with the speculative parallel
- * Thomas above: the same rows as k_adi_run (_calculate_preindex,
+ * Thomas of hd_thomas.h: the same rows as k_adi_run (_calculate_preindex,
  * _calculate_abcd, _calculate_matrix_coeff, tridiag, _extract_solution,
  * UVZSolver_multithread.cu:203-723) and the same bits.  Chunks of 255 cells
  * (510 rows) are assembled by the 64 lanes, four cells each (cell t = lane +
  * 64 h; the preindex of cell t + 1 comes from the neighbouring lane by a
- * shuffle), into LDS; the forward sweep and the back substitution run over
- * the chunk on all lanes; factors of all but the last chunk go to the run's
- * slice of the Ap/Bp line buffers and come back for back substitution. */
+ * shuffle), into LDS; the forward sweep (factors replacing C / D) and the back
+ * substitution (x replacing A) run over the chunk on all lanes; factors of
+ * all but the last chunk go to the run's slice of the Ap/Bp line buffers and
+ * come back for back substitution.  The run's wall-flag depths, boundary
+ * depths, edge values and the first cells' preindex inputs are loaded
+ * together, so a short run costs one memory round trip after its table
+ * entry. */
 template <bool isU>
-__global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, int shift)
+__global__ void __launch_bounds__(64, 3) k_adi_spec(const HdDev* __restrict__ dv, int shift)
 {
     const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
     constexpr int CH = 255; /* cells per LDS chunk (the rows of cell t need cells t and t + 1) */
     constexpr int RW = 2 * CH;
-    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];  /* rows; sA holds x after the forward sweep */
-    __shared__ double sAp[RW], sBp[RW];
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];
     __shared__ double sE[6];
     __shared__ double sX[3]; /* X(0), X(sn), X(last interior u row) */
     const int lane = threadIdx.x;
@@ -1087,22 +1119,40 @@ __global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, i
     const bool dkQ2 = isU ? d.bienQ[3] != 0 : d.bienQ[0] != 0;
     const int dkfr = isU ? N : M;
     const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
+    const double ND = d.P.NANGDAY;
     double* tz = d.t_z;
     double* tq = isU ? d.t_u : d.t_v;
+    const double* hh = d.h;
     const int nbr = (lane + 1) & (kWave - 1);
     for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
         if (line < sidx || line > eidx) continue;
         auto at = [&](int k) -> size_t { return isU ? (size_t)k * W + line : (size_t)line * W + k; };
-        bool br1, br2;
-        wall_flags(d, isU, line, first, last, br1, br2);
-        int skind, ekind, sn;
-        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
-        const double ub1 = isU ? d.ubt[line] : d.vbd[line];
-        const double ub2 = isU ? d.ubp[line] : d.vbt[line];
+        /* one round of loads: wall-flag depths (locate_segment_u / _v,
+         * constant.cuh:135-178), boundary depths, edge values, first cells */
+        double hb0, hb1, he0, he1;
+        if (isU) {
+            hb0 = hh[1 * W + line]; hb1 = hh[1 * W + line - 1];
+            he0 = hh[N * W + line]; he1 = hh[N * W + line - 1];
+        } else {
+            hb0 = hh[line * W + first - 1]; hb1 = hh[(line - 1) * W + first - 1];
+            he0 = hh[line * W + last]; he1 = hh[(line - 1) * W + last];
+        }
         const double TZ_l = tz[at(first)];
         const double TZ_r = tz[at(last)];
+        const double ub1 = isU ? d.ubt[line] : d.vbd[line];
+        const double ub2 = isU ? d.ubp[line] : d.vbt[line];
+        PreRaw raw0;
+        {
+            const int k = first + lane <= last ? first + lane : last;
+            raw0 = isU ? pre_uz_ld(d, k, line) : pre_vz_ld(d, line, k);
+        }
+        const bool br1 = (first > 2) || ((first == 2) && ((hb0 + hb1) * 0.5 == ND));
+        const bool br2 = isU ? (last < N) || ((last == N) && ((he0 + he1) * 0.5 == ND))
+                             : (last < M) || ((last == M) && ((he0 + he1) * 0.5 == ND));
+        int skind, ekind, sn;
+        run_kinds(br1, br2, dkQ1, dkQ2, first, last, dkfr, skind, ekind, sn);
         const int ru = (skind != S_OPEN) ? 2 * (last - 1 - first) + 1 : 2 * (last - 1 - first);
         double* gAp = d.Ap + (size_t)line * tw + 2 * first;
         double* gBp = d.Bp + (size_t)line * tw + 2 * first;
@@ -1113,37 +1163,45 @@ __global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, i
                 const int c0 = first + CH * c;
                 const int rb = RW * c;
                 const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
-                Pre pa[4] = {};
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    int k = c0 + lane + kWave * h;
-                    if (k <= last) {
-                        Pre a = isU ? pre_uz(d, k, line) : pre_vz(d, line, k);
-                        pa[h] = a;
-                        if (k == first) { sE[0] = a.a2; sE[1] = a.c2; sE[2] = a.d2; }
-                        if (k == last) { sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2; }
+                /* cells t = lane + 64 h; the loads of h + 1 are issued before
+                 * the rows of h are computed */
+                auto eval_cell = [&](int h, bool have_raw) -> Pre {
+                    const int k = c0 + lane + kWave * h;
+                    PreRaw r;
+                    if (have_raw) r = raw0;
+                    else {
+                        const int kc = k <= last ? k : last;
+                        r = isU ? pre_uz_ld(d, kc, line) : pre_vz_ld(d, line, kc);
                     }
-                }
-                Pre nb[4];
-#pragma unroll
-                for (int h = 0; h < 4; ++h) nb[h] = shfl_pre(pa[h], nbr);
+                    const Pre a = isU ? pre_uz_ev(d.P, r) : pre_vz_ev(d.P, r);
+                    if (k == first) { sE[0] = a.a2; sE[1] = a.c2; sE[2] = a.d2; }
+                    if (k == last) { sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2; }
+                    return a;
+                };
+                Pre cur = eval_cell(0, c == 0);
 #pragma unroll
                 for (int h = 0; h < 4; ++h) {
-                    int t = lane + kWave * h;
-                    int k = c0 + t;
+                    if (c0 + kWave * h > last) break; /* uniform */
+                    const bool more = h < 3 && c0 + kWave * (h + 1) <= last;
+                    Pre nxt = cur;
+                    if (more) nxt = eval_cell(h + 1, false);
+                    const Pre nA = shfl_pre(cur, nbr); /* cell t + 1 of lanes 0 .. 62 */
+                    const Pre nB = shfl_pre(nxt, 0);   /* cell t + 1 of lane 63 */
+                    const int t = lane + kWave * h;
+                    const int k = c0 + t;
                     if (t < CH && k <= last) {
-                        const Pre& a = pa[h];
-                        Pre b = a;
-                        if (k + 1 <= last) b = (lane < kWave - 1) ? nb[h] : nb[h < 3 ? h + 1 : 3];
+                        Pre b = cur;
+                        if (k + 1 <= last) b = (lane < kWave - 1) ? nA : nB;
                         for (int e = 0; e < 2; ++e) {
                             int rr = 2 * (k - first) + e;
                             if (rr > sn) break;
-                            Row4 w = cell_row(a, b, f4, k, e, first, last, br1, br2, skind, ekind, sn, ub1, ub2, TZ_l,
+                            Row4 w = cell_row(cur, b, f4, k, e, first, last, br1, br2, skind, ekind, sn, ub1, ub2, TZ_l,
                                               TZ_r);
                             int ix = rr - rb;
                             sA[ix] = w.A; sB[ix] = w.B; sC[ix] = w.C; sD[ix] = w.D;
                         }
                     }
+                    cur = nxt;
                 }
                 __syncthreads();
                 int g0 = 0;
@@ -1153,19 +1211,23 @@ __global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, i
                     } else {
                         ap = -sC[0] / sB[0];
                         bp = sD[0] / sB[0];
-                        if (lane == 0) { sAp[0] = ap; sBp[0] = bp; }
                     }
                     g0 = 1;
                 }
                 const int g1 = (re < sn - 1 ? re : sn - 1) - rb;
-                spec_forward(sA, sB, sC, sD, sAp, sBp, g0, g1, ap, bp);
+                spec_forward(sA, sB, sC, sD, g0, g1, ap, bp);
+                if (c == 0 && sn > 0) {
+                    __syncthreads();
+                    if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }
+                }
                 if (re == sn && sn > 0) {
                     double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
                     xs = (D - (A * bp)) / (B + (A * ap));
                 }
                 __syncthreads();
                 if (c + 1 < nch) { /* factors of this chunk (rows rb .. re, all < sn) to the line buffers */
-                    for (int i = lane; i <= re - rb; i += kWave) { gAp[rb + i] = sAp[i]; gBp[rb + i] = sBp[i]; }
+                    for (int i = lane; i <= re - rb; i += kWave) { gAp[rb + i] = sC[i]; gBp[rb + i] = sD[i]; }
+                    __syncthreads();
                 }
             }
             /* back substitution, last chunk first; x[r] replaces A[r] */
@@ -1175,13 +1237,13 @@ __global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, i
                 const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
                 if (c + 1 < nch) {
                     __syncthreads();
-                    for (int i = lane; i <= re - rb; i += kWave) { sAp[i] = gAp[rb + i]; sBp[i] = gBp[rb + i]; }
+                    for (int i = lane; i <= re - rb; i += kWave) { sC[i] = gAp[rb + i]; sD[i] = gBp[rb + i]; }
                     __syncthreads();
                 } else if (lane == 0) {
                     sA[sn - rb] = xs;
                 }
                 const int hi = (re < sn - 1 ? re : sn - 1) - rb;
-                xc = spec_backward(sAp, sBp, sA, hi, xc);
+                xc = spec_backward(sC, sD, sA, hi, xc);
                 __syncthreads();
                 /* scatter the unknowns of positions first .. last-1 */
                 for (int i = lane; i <= re - rb; i += kWave) {
@@ -1199,11 +1261,14 @@ __global__ void __launch_bounds__(64) k_adi_spec(const HdDev* __restrict__ dv, i
                     if (r == ru) sX[2] = x;
                 }
             }
-        } else if (lane == 0) {
+        } else {
             /* no system (one-cell run at an open start edge): the cell's
-             * preindex is still needed at the end */
-            Pre a = isU ? pre_uz(d, last, line) : pre_vz(d, line, last);
-            sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2;
+             * preindex is still needed at the end (first == last: lane 0
+             * loaded it) */
+            if (lane == 0) {
+                Pre a = isU ? pre_uz_ev(d.P, raw0) : pre_vz_ev(d.P, raw0);
+                sE[3] = a.a2; sE[4] = a.c2; sE[5] = a.d2;
+            }
         }
         __syncthreads();
         if (lane == 0) {
@@ -1270,20 +1335,39 @@ __global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv,
     const double* val = isU ? d.t_u : d.t_v;
     double* out = isU ? d.tmp_u : d.tmp_v;
     int close = isU ? W : 1, far = isU ? 1 : W;
-    double r = val[p];
-    if (i >= 2 && i <= N && j >= 2 && j <= M && r != 0) {
+    /* every value the cell may read, in one round of loads (neighbour
+     * indices fall back to the cell itself outside the ranges that use them) */
+    const bool in = i >= 2 && i <= N && j >= 2 && j <= M;
+    const double r0 = val[p];
+    const double vcm = val[in ? p - close : p], vcp = val[in ? p + close : p];
+    const double vfm = val[in ? p - far : p], vfp = val[in ? p + far : p];
+    const int kfm = d.khouot[in ? p - far : p], kfp = d.khouot[in ? p + far : p];
+    int kh = 0, kcm = 0, kcp = 0;
+    double htz = 0, tz = 0, zz = 0;
+    bool cin = false;
+    if (post) {
+        kh = d.khouot[p];
+        htz = d.htaiz[p];
+        tz = d.t_z[p];
+        zz = d.z[p];
+        cin = horizontal ? (i >= 1 && i <= N && j >= 2 && j <= M) : (j >= 1 && j <= M && i >= 2 && i <= N);
+        const int step = horizontal ? 1 : W;
+        kcm = d.khouot[cin ? p - step : p];
+        kcp = d.khouot[cin ? p + step : p];
+    }
+    double r = r0;
+    if (in && r != 0) {
         int count = 2;
-        double nb = val[p - close] + val[p + close];
-        if (d.khouot[p - far] == 0) { nb += val[p - far]; count++; }
-        if (d.khouot[p + far] == 0) { nb += val[p + far]; count++; }
+        double nb = vcm + vcp;
+        if (kfm == 0) { nb += vfm; count++; }
+        if (kfp == 0) { nb += vfp; count++; }
         r = r * d.P.heso + (1 - d.P.heso) * nb / count;
     }
     out[p] = r;
     if (!post) return;
-    int kh = d.khouot[p];
     unsigned char dry = 0;
     if (i >= 1 && i <= N && j >= 1 && j <= M && kh == 0) {
-        double hm = d.htaiz[p] + d.t_z[p];
+        double hm = htz + tz;
         d.H_moi[p] = hm;
         bool inrange = horizontal ? (j >= 2) : (i >= 2);
         dry = inrange && (hm <= d.P.H_TINH);
@@ -1292,14 +1376,11 @@ __global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv,
     /* a position can open a neighbour only where t_z > z; t_z changes
      * during the wetting pass only at cells it opens, and those are reached
      * by the pass's own cascade */
-    bool rises = d.t_z[p] > d.z[p];
-    if (horizontal) {
-        if (i >= 1 && i <= N && j >= 2 && j <= M && rises && (d.khouot[p - 1] == 1 || d.khouot[p + 1] == 1)) {
+    if (cin && tz > zz && (kcm == 1 || kcp == 1)) {
+        if (horizontal) {
             int slot = atomicAdd(&cand_cnt[i], 1);
             if (slot < kCand) cand[i * kCand + slot] = j;
-        }
-    } else {
-        if (j >= 1 && j <= M && i >= 2 && i <= N && rises && (d.khouot[p - W] == 1 || d.khouot[p + W] == 1)) {
+        } else {
             int slot = atomicAdd(&cand_cnt[j], 1);
             if (slot < kCand) cand[j * kCand + slot] = i;
         }
@@ -1308,81 +1389,116 @@ __global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv,
 
 /* ================================================================ explicit */
 
-/* vSolver, UVZSolver_multithread.cu:727-768, at (row, col); reads the
- * smoothed t_u. */
-__device__ double v_solver(const HdDevG& d, int row, int col)
+/* Every value vSolver / uSolver may read at one cell, loaded
+ * unconditionally (all in bounds for the cells the explicit kernels visit)
+ * so that the loads of several cells are in flight together; the
+ * evaluations below take the reference's branches on them. */
+struct ExRaw {
+    double c, a1, b1;              /* the velocity at the cell and at +-1 along its own direction */
+    double n1, n2, m1, m2;         /* the velocity at +1, +2, -1, -2 across */
+    double x0, x1, x2, x3;         /* the other velocity at the four surrounding faces */
+    double t0, t1, t2, t3;         /* the same from its new (smoothed) field */
+    double ht, k1, hm_m1, hm_p1, hm_m2, zf, zc, vis, ts;
+};
+
+/* loads of vSolver (UVZSolver_multithread.cu:727-768) at (row, col) */
+__device__ __forceinline__ ExRaw v_solver_ld(const HdDevG& d, int row, int col)
 {
-    const hd_params& P = d.P;
-    int W = d.W;
-    const double *v = d.v, *u = d.u, *t_u = d.t_u, *z = d.z, *Hm = d.H_moi;
+    const int W = d.W, p = row * W + col;
+    ExRaw r;
+    r.c = d.v[p]; r.a1 = d.v[p + 1]; r.b1 = d.v[p - 1];
+    r.n1 = d.v[p + W]; r.n2 = d.v[p + 2 * W]; r.m1 = d.v[p - W]; r.m2 = d.v[p - 2 * W];
+    r.x0 = d.u[p - W]; r.x1 = d.u[p]; r.x2 = d.u[p - W + 1]; r.x3 = d.u[p + 1];
+    r.t0 = d.t_u[p - W]; r.t1 = d.t_u[p]; r.t2 = d.t_u[p - W + 1]; r.t3 = d.t_u[p + 1];
+    r.ht = d.Htdv[p]; r.k1 = d.Ky1[p];
+    r.hm_m1 = d.H_moi[p - W]; r.hm_p1 = d.H_moi[p + W]; r.hm_m2 = d.H_moi[p - 2 * W];
+    r.zf = d.z[p + 1]; r.zc = d.z[p]; r.vis = d.VISCOIDX[p]; r.ts = d.Tsyw[p];
+    return r;
+}
+
+/* vSolver, UVZSolver_multithread.cu:727-768, on the loaded values; reads the
+ * smoothed t_u. */
+__device__ __forceinline__ double v_solver_ev(const hd_params& P, const ExRaw& r)
+{
     double p, q, tmp;
     q = 0; p = 0; tmp = 0;
-    double vc = v[row * W + col];
-    double utb = (u[(row - 1) * W + col] + u[row * W + col] + u[(row - 1) * W + col + 1] + u[row * W + col + 1]) * 0.25;
-    double t_utb = (t_u[(row - 1) * W + col] + t_u[row * W + col] + t_u[(row - 1) * W + col + 1] + t_u[row * W + col + 1]) * 0.25;
-    double htv = d.Htdv[row * W + col];
-    p = (v[row * W + col + 1] - v[row * W + col - 1]) / P.dY2;
-    p = P.HaiChiadT + p + d.Ky1[row * W + col] * sqrt(utb * utb + vc * vc) / htv;
-    if (Hm[(row - 1) * W + col] <= P.H_TINH) {
+    const double vc = r.c;
+    const double utb = (r.x0 + r.x1 + r.x2 + r.x3) * 0.25;
+    const double t_utb = (r.t0 + r.t1 + r.t2 + r.t3) * 0.25;
+    const double htv = r.ht;
+    p = (r.a1 - r.b1) / P.dY2;
+    p = P.HaiChiadT + p + r.k1 * sqrt(utb * utb + vc * vc) / htv;
+    if (r.hm_m1 <= P.H_TINH) {
         if (utb < 0) {
-            q = t_utb * (-3 * vc + 4 * v[(row + 1) * W + col] - v[(row + 2) * W + col]) / P.dX2;
-            tmp = (vc - 2 * v[(row + 1) * W + col] + v[(row + 2) * W + col]) / P.dXbp;
+            q = t_utb * (-3 * vc + 4 * r.n1 - r.n2) / P.dX2;
+            tmp = (vc - 2 * r.n1 + r.n2) / P.dXbp;
         }
     } else {
-        if (Hm[(row + 1) * W + col] <= P.H_TINH) {
-            if ((Hm[(row - 2) * W + col] > P.H_TINH) && (utb > 0)) {
-                q = t_utb * (3 * vc - 4 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / P.dX2;
-                tmp = (vc - 2 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / P.dXbp;
+        if (r.hm_p1 <= P.H_TINH) {
+            if ((r.hm_m2 > P.H_TINH) && (utb > 0)) {
+                q = t_utb * (3 * vc - 4 * r.m1 + r.m2) / P.dX2;
+                tmp = (vc - 2 * r.m1 + r.m2) / P.dXbp;
             }
         } else {
-            q = t_utb * (v[(row + 1) * W + col] - v[(row - 1) * W + col]) / P.dX2;
-            tmp = (v[(row + 1) * W + col] - 2 * vc + v[(row - 1) * W + col]) / P.dXbp;
+            q = t_utb * (r.n1 - r.m1) / P.dX2;
+            tmp = (r.n1 - 2 * vc + r.m1) / P.dXbp;
         }
     }
     q = P.HaiChiadT * vc - q - P.CORIOLIS_FORCE * t_utb;
-    q = (q - P.g * (z[row * W + col + 1] - z[row * W + col]) / P.dY
-         + d.VISCOIDX[row * W + col] * (tmp + (v[row * W + col + 1] - 2 * vc + v[row * W + col - 1]) / P.dYbp))
-        + (P.Windy - d.Tsyw[row * W + col]) / htv;
+    q = (q - P.g * (r.zf - r.zc) / P.dY + r.vis * (tmp + (r.a1 - 2 * vc + r.b1) / P.dYbp)) + (P.Windy - r.ts) / htv;
     return q / p;
 }
 
-/* uSolver, UVZSolver_multithread.cu:773-815, at (row, col); reads the
- * smoothed t_v. */
-__device__ double u_solver(const HdDevG& d, int row, int col)
+/* loads of uSolver (UVZSolver_multithread.cu:773-815) at (row, col) */
+__device__ __forceinline__ ExRaw u_solver_ld(const HdDevG& d, int row, int col)
 {
-    const hd_params& P = d.P;
-    int W = d.W;
-    const double *v = d.v, *u = d.u, *t_v = d.t_v, *z = d.z, *Hm = d.H_moi;
+    const int W = d.W, p = row * W + col;
+    ExRaw r;
+    r.c = d.u[p]; r.a1 = d.u[p + W]; r.b1 = d.u[p - W];
+    r.n1 = d.u[p + 1]; r.n2 = d.u[p + 2]; r.m1 = d.u[p - 1]; r.m2 = d.u[p - 2];
+    r.x0 = d.v[p - 1]; r.x1 = d.v[p]; r.x2 = d.v[p + W - 1]; r.x3 = d.v[p + W];
+    r.t0 = d.t_v[p - 1]; r.t1 = d.t_v[p]; r.t2 = d.t_v[p + W - 1]; r.t3 = d.t_v[p + W];
+    r.ht = d.Htdu[p]; r.k1 = d.Kx1[p];
+    r.hm_m1 = d.H_moi[p - 1]; r.hm_p1 = d.H_moi[p + 1]; r.hm_m2 = d.H_moi[p - 2];
+    r.zf = d.z[p + W]; r.zc = d.z[p]; r.vis = d.VISCOIDX[p]; r.ts = d.Tsxw[p];
+    return r;
+}
+
+/* uSolver, UVZSolver_multithread.cu:773-815, on the loaded values; reads the
+ * smoothed t_v. */
+__device__ __forceinline__ double u_solver_ev(const hd_params& P, const ExRaw& r)
+{
     double p, q, tmp;
     p = 0; q = 0; tmp = 0;
-    double uc = u[row * W + col];
-    double vtb = (v[row * W + col - 1] + v[row * W + col] + v[(row + 1) * W + col - 1] + v[(row + 1) * W + col]) * 0.25;
-    double t_vtb = (t_v[row * W + col - 1] + t_v[row * W + col] + t_v[(row + 1) * W + col - 1] + t_v[(row + 1) * W + col]) * 0.25;
-    double htu = d.Htdu[row * W + col];
-    p = (u[(row + 1) * W + col] - u[(row - 1) * W + col]) / P.dX2;
-    p = (P.HaiChiadT + p + d.Kx1[row * W + col] * sqrt(vtb * vtb + uc * uc) / htu);
-    if (Hm[row * W + col - 1] <= P.H_TINH) {
+    const double uc = r.c;
+    const double vtb = (r.x0 + r.x1 + r.x2 + r.x3) * 0.25;
+    const double t_vtb = (r.t0 + r.t1 + r.t2 + r.t3) * 0.25;
+    const double htu = r.ht;
+    p = (r.a1 - r.b1) / P.dX2;
+    p = (P.HaiChiadT + p + r.k1 * sqrt(vtb * vtb + uc * uc) / htu);
+    if (r.hm_m1 <= P.H_TINH) {
         if (vtb < 0) {
-            q = t_vtb * (-3 * uc + 4 * u[row * W + col + 1] - u[row * W + col + 2]) / P.dY2;
-            tmp = (uc - 2 * u[row * W + col + 1] + u[row * W + col + 2]) / P.dYbp;
+            q = t_vtb * (-3 * uc + 4 * r.n1 - r.n2) / P.dY2;
+            tmp = (uc - 2 * r.n1 + r.n2) / P.dYbp;
         }
     } else {
-        if (Hm[row * W + col + 1] <= P.H_TINH) {
-            if ((Hm[row * W + col - 2] > P.H_TINH) && (vtb > 0)) {
-                q = t_vtb * (3 * uc - 4 * u[row * W + col - 1] + u[row * W + col - 2]) / P.dY2;
-                tmp = (uc - 2 * u[row * W + col - 1] + u[row * W + col - 2]) / P.dYbp;
+        if (r.hm_p1 <= P.H_TINH) {
+            if ((r.hm_m2 > P.H_TINH) && (vtb > 0)) {
+                q = t_vtb * (3 * uc - 4 * r.m1 + r.m2) / P.dY2;
+                tmp = (uc - 2 * r.m1 + r.m2) / P.dYbp;
             }
         } else {
-            q = t_vtb * (u[row * W + col + 1] - u[row * W + col - 1]) / P.dY2;
-            tmp = (u[row * W + col + 1] - 2 * uc + u[row * W + col - 1]) / P.dYbp;
+            q = t_vtb * (r.n1 - r.m1) / P.dY2;
+            tmp = (r.n1 - 2 * uc + r.m1) / P.dYbp;
         }
     }
     q = P.HaiChiadT * uc - q + P.CORIOLIS_FORCE * t_vtb;
-    q = (q - P.g * (z[(row + 1) * W + col] - z[row * W + col]) / P.dX
-         + d.VISCOIDX[row * W + col] * (tmp + (u[(row + 1) * W + col] - 2 * uc + u[(row - 1) * W + col]) / P.dXbp))
-        + (P.Windx - d.Tsxw[row * W + col]) / htu;
+    q = (q - P.g * (r.zf - r.zc) / P.dX + r.vis * (tmp + (r.a1 - 2 * uc + r.b1) / P.dXbp)) + (P.Windx - r.ts) / htu;
     return q / p;
 }
+
+__device__ double v_solver(const HdDevG& d, int row, int col) { return v_solver_ev(d.P, v_solver_ld(d, row, col)); }
+__device__ double u_solver(const HdDevG& d, int row, int col) { return u_solver_ev(d.P, u_solver_ld(d, row, col)); }
 
 /* solveV (UVZSolver_multithread.cu:1055-1071) or solveU (:947-968): one
  * thread per cell, the cell's run from an LDS copy of the line tables; cells
@@ -1490,7 +1606,7 @@ __global__ void k_margins(const HdDev* __restrict__ dv)
  * field and the old state, and a line's margins read only that line, so this
  * is the two-kernel order (all explicit cells, then margins) line by line. */
 template <bool isU>
-__global__ void __launch_bounds__(64) k_explicit_margins(const HdDev* __restrict__ dv)
+__global__ void __launch_bounds__(64) k_explicit_margins(const HdDev* __restrict__ dv, int cells)
 {
     const HdDevG& d = dev_view(dv);
     const int eidx = isU ? d.M : d.N;
@@ -1502,14 +1618,45 @@ __global__ void __launch_bounds__(64) k_explicit_margins(const HdDev* __restrict
     if (nseg > SL) nseg = SL;
     const int* dau = (isU ? d.dauj : d.daui) + line * SL;
     const int* cuoi = (isU ? d.cuoij : d.cuoii) + line * SL;
-    for (int sg = 0; sg < nseg; ++sg) {
-        const int first = dau[sg], last = cuoi[sg];
-        for (int k = first + lane; k < last; k += kWave) {
-            if (isU) {
-                if (k >= 1 && k <= N) d.t_u[k * W + line] = u_solver(d, k, line);
-            } else {
-                if (k <= M && line <= N) d.t_v[line * W + k] = v_solver(d, line, k);
-            }
+    /* the cells first .. last-1 of all runs of the line, numbered across the
+     * runs (run s starts at s_ex[s]); four cells per lane per round, all
+     * loads of a round in flight before any store */
+    __shared__ int s_ex[25], s_f[24];
+    int len = 0, f0 = 0;
+    if (lane < nseg) {
+        f0 = dau[lane];
+        const int l0 = cuoi[lane];
+        len = l0 > f0 ? l0 - f0 : 0;
+    }
+    int incl = len;
+    for (int dd = 1; dd < 32; dd <<= 1) {
+        const int v = __shfl_up(incl, dd, kWave);
+        if (lane >= dd) incl += v;
+    }
+    if (lane < nseg) { s_ex[lane] = incl - len; s_f[lane] = f0; }
+    const int total = __shfl(incl, nseg > 0 ? nseg - 1 : 0, kWave) * (nseg > 0);
+    __syncthreads();
+    constexpr int B = 4;
+    for (int base = 0; cells && base < total; base += B * kWave) {
+        int kk[B];
+        bool on[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int idx = base + lane + b * kWave;
+            int sg = 0;
+            while (sg + 1 < nseg && s_ex[sg + 1] <= idx) ++sg;
+            const int k = s_f[sg] + (idx - s_ex[sg]);
+            on[b] = idx < total && (isU ? (k >= 1 && k <= N) : (k <= M && line <= N));
+            kk[b] = on[b] ? k : s_f[0]; /* an inactive slot loads a cell of the line's first run */
+        }
+        ExRaw r[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) r[b] = isU ? u_solver_ld(d, kk[b], line) : v_solver_ld(d, line, kk[b]);
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (!on[b]) continue;
+            if (isU) d.t_u[kk[b] * W + line] = u_solver_ev(d.P, r[b]);
+            else d.t_v[line * W + kk[b]] = v_solver_ev(d.P, r[b]);
         }
     }
     __syncthreads();
@@ -1539,6 +1686,29 @@ __global__ void __launch_bounds__(64) k_explicit_margins(const HdDev* __restrict
             else t[i * W + last] = 2 * t[i * W + last - 1] - t[i * W + last - 2];
         }
     }
+}
+
+/* solveU (UVZSolver_multithread.cu:947-968) over the active row tiles: the
+ * cells first .. last-1 of every wet run of every column.  They are
+ * recognised from the mask the run tables were built from (khouot changes
+ * only later in the half-step): along a column of positions 2 .. N, cell k
+ * belongs to a run's first .. last-1 iff k and k+1 are wet (khouot == 0) and
+ * k's wet interval starts at or before N-2 -- for k <= N-2 always, for
+ * k = N-1 iff N-2 is wet -- which is how Find_Calculation_limits_Vertical
+ * (support_funcs.cu:252-279) forms the runs.  A wave's loads are then 64
+ * adjacent columns of one row instead of 64 rows of one column;
+ * k_explicit_margins (cells off) writes the margins after it. */
+__global__ void __launch_bounds__(256) k_explicit_rows_u(const HdDev* __restrict__ dv, int tiled)
+{
+    const HdDevG& d = dev_view(dv);
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
+    const int N = d.N, M = d.M, W = d.W;
+    if (j < 2 || j > M || i < 2 || i > N - 1) return;
+    const int* kh = d.khouot;
+    const int p = i * W + j;
+    const bool in = kh[p] == 0 && kh[p + W] == 0 && (i <= N - 2 || kh[(N - 2) * W + j] == 0);
+    if (in) d.t_u[p] = u_solver(d, i, j);
 }
 
 /* ================================================================ wet / dry */
@@ -1781,7 +1951,7 @@ __global__ void k_reset_runs(const HdDev* __restrict__ dv)
  * number of runs found. */
 template <bool ROW>
 __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, int SL, int lane, int* dau,
-                                         int* cuoi, int* sf, int* sl)
+                                         int* cuoi, int* sf, int* sl, int2 ext)
 {
     auto wetv = [&](int x) -> int {
         return (x >= 2 && x <= L) ? (kh[ROW ? line * W + x : x * W + line] == 0) : 0;
@@ -1789,15 +1959,18 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
     const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
     int ns = 0, ne = 0;
     int carry = 0; /* wet(first position of the chunk - 1) */
-    constexpr int kB = 8; /* chunks of 64 cells per batch, all loads in flight together */
-    for (int b0 = 2; b0 <= L; b0 += kB * kWave) {
+    constexpr int kB = 16; /* chunks of 64 cells per batch, all loads in flight together */
+    /* only positions inside the line's extent can be wet; before it the
+     * previous position is dry, as carry says */
+    const int x0 = ext.x > 2 ? ext.x : 2, x1 = ext.y < L ? ext.y : L;
+    for (int b0 = x0; b0 <= x1; b0 += kB * kWave) {
         int w[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) w[u] = wetv(b0 + kWave * u + lane);
         const int after = wetv(b0 + kB * kWave); /* the cell after the batch */
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            if (b0 + kWave * u > L) break;
+            if (b0 + kWave * u > x1) break;
             const int x = b0 + kWave * u + lane;
             const int up = __shfl_up(w[u], 1, kWave);
             const int dn = __shfl_down(w[u], 1, kWave);
@@ -1827,21 +2000,82 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
 
 /* Interpolate_FS_ng rows N and 2, then Interpolate_FS_d column M, by one
  * workgroup (the two reference kernels' order; the column reads the rows'
- * corner results). */
+ * corner results).  Each edge cell touches only its own FS entry and cells
+ * of its own line, so the cells of one edge are independent.  The velocity
+ * loads of all three edges go out together; per edge the FS loads follow
+ * (one round trip), with a barrier between the edges. */
 __device__ void fs_edges_block(const HdDevG& d)
 {
-    int N = d.N, M = d.M;
-    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
-        fs_ng(d, N, N - 1, -1, j);
+    const int N = d.N, M = d.M, W = d.W;
+    const int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
+    const int T = blockDim.x, tid = threadIdx.x;
+    constexpr int K = 8;
+    if (M - 1 > K * T || imax > K * T) { /* very wide grid: edge by edge */
+        for (int j = 1 + tid; j < M; j += T) fs_ng(d, N, N - 1, -1, j);
+        __syncthreads();
+        for (int j = 1 + tid; j < M; j += T) fs_ng(d, 2, 2, 1, j);
+        __syncthreads();
+        for (int i = 1 + tid; i <= imax; i += T) fs_d(d, M, M - 1, -1, i);
+        return;
     }
-    for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
-        fs_ng(d, 2, 2, 1, j);
+    const double dT = d.P.dT, dX = d.P.dX, dY = d.P.dY;
+    double dl[3][K];
+    bool on[3][K];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            const int x = 1 + tid + b * T;
+            double tq = 0, q = 0;
+            bool valid;
+            if (p < 2) {
+                const int loc = p == 0 ? N : 2;
+                valid = x < M;
+                if (valid) { tq = d.t_u[loc * W + x]; q = d.u[loc * W + x]; }
+            } else {
+                valid = x <= imax;
+                if (valid) { tq = d.t_v[x * W + M]; q = d.v[x * W + M]; }
+            }
+            on[p][b] = valid && tq > 0;
+            dl[p][b] = (q + tq) * 0.25 * dT;
+        }
     }
-    __syncthreads();
-    int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
-    for (int i = 1 + threadIdx.x; i <= imax; i += blockDim.x) fs_d(d, M, M - 1, -1, i);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) { /* Interpolate_FS_ng, support_funcs.cu:189-208 */
+        const int location = p == 0 ? N : 2, offset = p == 0 ? N - 1 : 2, sign = p == 0 ? -1 : 1;
+        const int i_dX = hd_rint_int(dX);
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            if (!on[p][b]) continue;
+            const int j = 1 + tid + b * T;
+            const double deltaX = dl[p][b];
+            int i = (int)(offset + sign * (deltaX / dX));
+            if (i < 0) i = 0;
+            if (i > N + 1) i = N + 1;
+            const int i_deltaX = hd_rint_int(deltaX);
+            const int q = i_dX != 0 ? i_deltaX / i_dX : 0;
+            const double f0 = d.FS[i * W + j], f1 = d.FS[(i + 1) * W + j];
+            d.FS[location * W + j] = f0 + sign * (f1 - f0) * (deltaX / dX - q);
+        }
+        __syncthreads();
+    }
+    {   /* Interpolate_FS_d, support_funcs.cu:210-229, column M */
+        const int j_dY = hd_rint_int(dY);
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            if (!on[2][b]) continue;
+            const int i = 1 + tid + b * T;
+            const double deltaY = dl[2][b];
+            int j = (int)((M - 1) + -1 * (deltaY / dY));
+            if (j < 0) j = 0;
+            if (j > M + 1) j = M + 1;
+            const int j_deltaY = hd_rint_int(deltaY);
+            const int q = j_dY != 0 ? j_deltaY / j_dY : 0;
+            const double f0 = d.FS[i * W + j], f1 = d.FS[i * W + j + 1];
+            d.FS[i * W + M] = f1 + (f1 - f0) * (deltaY / dY - q);
+        }
+    }
 }
-
 
 /* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335):
  * the per-line tables daui / cuoii / moci (rows i = 2..N) and dauj / cuoij /
@@ -1870,12 +2104,14 @@ __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ d
         line = i;
         if (count_wet && i <= N + 2) {
             int c = 0;
-            for (int j = lane; j < M + 3; j += kWave) c += (kh[i * W + j] == 0);
+            const int2 e = d.ext[i];
+            const int hi = e.y < M + 2 ? e.y : M + 2;
+            for (int j = e.x + lane; j <= hi; j += kWave) c += (kh[i * W + j] == 0);
             for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
             if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
         }
         if (i >= 2 && i <= N) {
-            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv]);
+            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv], d.ext[i]);
             n = ns < SL ? ns : SL;
             if (lane == 0) {
                 d.moci[i] = n;
@@ -1886,7 +2122,8 @@ __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ d
         const int j = 2 + 4 * (blockIdx.x - nrowb) + wv;
         line = j;
         if (j <= M) {
-            int ns = scan_runs<false>(kh, W, j, N, SL, lane, d.dauj + j * SL, d.cuoij + j * SL, s_f[wv], s_l[wv]);
+            int ns = scan_runs<false>(kh, W, j, N, SL, lane, d.dauj + j * SL, d.cuoij + j * SL, s_f[wv], s_l[wv],
+                                      d.ext[N + 3 + j]);
             n = ns < SL ? ns : SL;
             if (lane == 0) {
                 d.mocj[j] = n;
@@ -2051,11 +2288,16 @@ void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s)
     else hipLaunchKernelGGL(k_margins<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
-void hd_launch_explicit_margins(const HdBatch& b, int isU, hipStream_t s)
+void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_t s)
 {
     int nl = (isU ? b.M : b.N) - 1;
-    if (isU) hipLaunchKernelGGL(k_explicit_margins<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv);
-    else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv);
+    if (isU) hipLaunchKernelGGL(k_explicit_margins<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
+    else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
+}
+
+void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_explicit_rows_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
 }
 
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)

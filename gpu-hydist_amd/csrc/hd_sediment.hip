@@ -509,8 +509,7 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
 {
     const HdDevG& d = dev_view(dv);
     constexpr int RW = 512;
-    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW]; /* rows; sA holds x after the forward sweep */
-    __shared__ double sAp[RW], sBp[RW];
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW]; /* rows; factors replace C / D, x replaces A */
     __shared__ double sV[2]; /* clamped x of the first and last row */
     const int N = d.N, M = d.M, W = d.W;
     const int lane = threadIdx.x;
@@ -550,19 +549,24 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
                 } else {
                     ap = -sC[0] / sB[0];
                     bp = sD[0] / sB[0];
-                    if (lane == 0) { sAp[0] = ap; sBp[0] = bp; }
                 }
                 g0 = 1;
             }
             const int g1 = ((re < sn - 1) ? re : sn - 1) - rb;
-            hd_spec::spec_forward(sA, sB, sC, sD, sAp, sBp, g0, g1, ap, bp);
+            hd_spec::spec_forward(sA, sB, sC, sD, g0, g1, ap, bp);
+            if (c == 0 && sn > 0) {
+                __syncthreads();
+                if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }
+            }
             if (re == sn && sn > 0) {
                 double A = sA[sn - rb], B = sB[sn - rb], D = sD[sn - rb];
                 xs = (D - (A * bp)) / (B + (A * ap));
             }
             __syncthreads();
-            if (c + 1 < nch)
-                for (int q = lane; q <= re - rb; q += 64) { gAp[rb + q] = sAp[q]; gBp[rb + q] = sBp[q]; }
+            if (c + 1 < nch) {
+                for (int q = lane; q <= re - rb; q += 64) { gAp[rb + q] = sC[q]; gBp[rb + q] = sD[q]; }
+                __syncthreads();
+            }
         }
         double xc = xs;
         for (int c = nch - 1; c >= 0; --c) {
@@ -570,13 +574,13 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
             const int re = (rb + RW - 1 < sn) ? rb + RW - 1 : sn;
             if (c + 1 < nch) {
                 __syncthreads();
-                for (int q = lane; q <= re - rb; q += 64) { sAp[q] = gAp[rb + q]; sBp[q] = gBp[rb + q]; }
+                for (int q = lane; q <= re - rb; q += 64) { sC[q] = gAp[rb + q]; sD[q] = gBp[rb + q]; }
                 __syncthreads();
             } else if (lane == 0) {
                 sA[sn - rb] = xs;
             }
             const int hi = ((re < sn - 1) ? re : sn - 1) - rb;
-            xc = hd_spec::spec_backward(sAp, sBp, sA, hi, xc);
+            xc = hd_spec::spec_backward(sC, sD, sA, hi, xc);
             __syncthreads();
             for (int q = lane; q <= re - rb; q += 64) {
                 double v = sA[q];

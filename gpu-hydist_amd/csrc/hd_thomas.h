@@ -41,6 +41,7 @@ constexpr int kSpecWF = 24;           /* forward warm-up rows after the estimate
 constexpr int kSpecWB = 32;           /* back-substitution warm-up rows */
 constexpr int kSpecSerialF = 48;      /* up to this many rows the forward sweep stays on one lane */
 constexpr int kSpecSerialB = 96;      /* likewise for back substitution */
+constexpr int kSpecMaxL = 8;          /* rows per lane of one speculative sweep (chunks <= 512 rows) */
 
 struct Mx {
     double a, b, c, d, e, f, g; /* [[a b 0] [c d 0] [e f g]] */
@@ -117,64 +118,92 @@ __device__ __forceinline__ void thomas_row(const double* sA, const double* sB, c
 
 /* Forward elimination of LDS rows g0 .. g1 (chunk-relative), entering with
  * the exact state (ap, bp) of row g0 - 1, which is left as the state of row
- * g1; the factors of the rows go to sAp / sBp.  One wave, uniform control. */
-__device__ __forceinline__ void spec_forward(const double* sA, const double* sB, const double* sC, const double* sD,
-                                             double* sAp, double* sBp, int g0, int g1, double& ap, double& bp)
+ * g1.  The factors ap / bp of the rows replace C / D (after every lane has
+ * read them: speculating lanes keep their own rows' factors in registers
+ * until the sweep is verified).  At most 64 * kSpecMaxL rows; one wave,
+ * uniform control; ends with a barrier. */
+__device__ __forceinline__ void spec_forward(const double* sA, const double* sB, double* sC, double* sD, int g0,
+                                             int g1, double& ap, double& bp)
 {
     const int lane = threadIdx.x;
     const int n = g1 - g0 + 1;
     if (n <= 0) return;
-    const int L = n <= kSpecSerialF ? n : (n + kSpecWave - 1) / kSpecWave;
+    if (n <= kSpecSerialF) {
+        /* short sweep: lane 0 alone, factors written in place */
+        double ca = ap, cb = bp;
+        if (lane == 0) {
+#pragma unroll 4
+            for (int i = g0; i <= g1; ++i) {
+                thomas_row(sA, sB, sC, sD, i, ca, cb);
+                sC[i] = ca;
+                sD[i] = cb;
+            }
+        }
+        ap = __shfl(ca, 0, kSpecWave);
+        bp = __shfl(cb, 0, kSpecWave);
+        __syncthreads();
+        return;
+    }
+    const int L = (n + kSpecWave - 1) / kSpecWave; /* rows per lane, <= kSpecMaxL */
     const int sk = g0 + lane * L;
     const int ek = sk + L <= g1 + 1 ? sk + L : g1 + 1;
     const bool act = sk <= g1;
     const int m = (kSpecWF + L - 1) / L;
     const int j = lane - m - 1;
-    double ca = ap, cb = bp;
-    int i0 = g0;
-    if (n > kSpecSerialF) {
-        /* prefix estimates */
-        Mx X{1, 0, 0, 1, 0, 0, 1};
-        for (int i = sk; i < ek; ++i) {
-            X = mx_row(sA[i], sB[i], sC[i], sD[i], X);
-            if (((i - sk) & 3) == 3) mx_norm(X);
+    /* prefix estimates */
+    Mx X{1, 0, 0, 1, 0, 0, 1};
+    for (int i = sk; i < ek; ++i) {
+        X = mx_row(sA[i], sB[i], sC[i], sD[i], X);
+        if (((i - sk) & 3) == 3) mx_norm(X);
+    }
+    mx_norm(X);
+    for (int dd = 1; dd < kSpecWave; dd <<= 1) {
+        Mx Y = mx_shfl_up(X, dd);
+        if (lane >= dd) {
+            X = mx_mul(X, Y);
+            mx_norm(X);
         }
-        mx_norm(X);
-        for (int dd = 1; dd < kSpecWave; dd <<= 1) {
-            Mx Y = mx_shfl_up(X, dd);
-            if (lane >= dd) {
-                X = mx_mul(X, Y);
-                mx_norm(X);
-            }
-        }
+    }
+    double ea, eb;
+    {
         double p = fma(X.a, ap, X.b), q = fma(X.c, ap, X.d), r = fma(X.e, ap, fma(X.g, bp, X.f));
-        double ea = p / q, eb = r / q;
-        int src = j < 0 ? 0 : j;
-        ea = __shfl(ea, src, kSpecWave);
-        eb = __shfl(eb, src, kSpecWave);
-        if (j >= 0) { ca = ea; cb = eb; i0 = g0 + (j + 1) * L; }
+        ea = __shfl(p / q, j < 0 ? 0 : j, kSpecWave);
+        eb = __shfl(r / q, j < 0 ? 0 : j, kSpecWave);
     }
     const bool exact = j < 0;
-    if (!act) i0 = ek;
-    double wa = ca, wb = cb;
-    for (int i = i0; i < ek; ++i) {
-        if (i == sk) { wa = ca; wb = cb; }
-        thomas_row(sA, sB, sC, sD, i, ca, cb);
-        if (i >= sk) { sAp[i] = ca; sBp[i] = cb; }
+    double ca = exact ? ap : ea, cb = exact ? bp : eb;
+    int i0 = exact ? g0 : g0 + (j + 1) * L;
+    if (!act) i0 = sk;
+    /* warm-up rows, then the lane's own rows */
+    for (int i = i0; i < sk; ++i) thomas_row(sA, sB, sC, sD, i, ca, cb);
+    const double wa = ca, wb = cb;
+    double ra[kSpecMaxL], rb[kSpecMaxL];
+#pragma unroll
+    for (int q = 0; q < kSpecMaxL; ++q) {
+        if (sk + q < ek) {
+            thomas_row(sA, sB, sC, sD, sk + q, ca, cb);
+            ra[q] = ca;
+            rb[q] = cb;
+        }
     }
-    if (n > kSpecSerialF) {
+    /* exact verification against the predecessor's last row */
+    {
         double pa = __shfl_up(ca, 1, kSpecWave), pb = __shfl_up(cb, 1, kSpecWave);
         bool ok = exact || !act || (same_bits(wa, pa) && same_bits(wb, pb));
         unsigned long long bad = __ballot(!ok);
         while (bad) {
             const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
-            __syncthreads();
+            double fa = __shfl(ca, f - 1, kSpecWave), fb = __shfl(cb, f - 1, kSpecWave);
             if (lane == f) {
-                ca = sAp[sk - 1];
-                cb = sBp[sk - 1];
-                for (int i = sk; i < ek; ++i) {
-                    thomas_row(sA, sB, sC, sD, i, ca, cb);
-                    sAp[i] = ca; sBp[i] = cb;
+                ca = fa;
+                cb = fb;
+#pragma unroll
+                for (int q = 0; q < kSpecMaxL; ++q) {
+                    if (sk + q < ek) {
+                        thomas_row(sA, sB, sC, sD, sk + q, ca, cb);
+                        ra[q] = ca;
+                        rb[q] = cb;
+                    }
                 }
             }
             double na = __shfl(ca, f, kSpecWave), nb = __shfl(cb, f, kSpecWave);
@@ -186,6 +215,15 @@ __device__ __forceinline__ void spec_forward(const double* sA, const double* sB,
     const int klast = (n - 1) / L;
     ap = __shfl(ca, klast, kSpecWave);
     bp = __shfl(cb, klast, kSpecWave);
+    __syncthreads(); /* every lane has read C / D */
+#pragma unroll
+    for (int q = 0; q < kSpecMaxL; ++q) {
+        if (sk + q < ek) {
+            sC[sk + q] = ra[q];
+            sD[sk + q] = rb[q];
+        }
+    }
+    __syncthreads();
 }
 
 /* Back substitution over LDS rows hi down to 0 (chunk-relative) given x of
@@ -221,12 +259,13 @@ __device__ __forceinline__ double spec_backward(const double* sAp, const double*
         if (j >= 0) { cx = est; i0 = hi - (j + 1) * L; }
     }
     const bool exact = j < 0;
-    if (!act) i0 = bot - 1;
-    double w = cx;
-    for (int i = i0; i >= bot; --i) {
-        if (i == top) w = cx;
+    if (!act) i0 = top;
+    for (int i = i0; i > top; --i) cx = sBp[i] + (sAp[i] * cx);
+    const double w = cx;
+#pragma unroll 8
+    for (int i = (top < i0 ? top : i0); i >= bot; --i) {
         cx = sBp[i] + (sAp[i] * cx);
-        if (i <= top) sX[i] = cx;
+        sX[i] = cx;
     }
     if (n > kSpecSerialB) {
         double px = __shfl_up(cx, 1, kSpecWave);
