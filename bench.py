@@ -55,6 +55,8 @@ KERNEL_BYTES = {
     "smooth_u_hmoi": 8 + 4 + 8 + 2 * 8 + 8,
     # wet-run tables: khouot read
     "segments_fs_edges": 4,
+    # the two in one launch (state copy + tables)
+    "uvz_segments": 5 * 8 + 5 * 8 + 4,
     # the same work, one wave per line (HYDIST_SOLVER=line)
     "adi_line_uz": 9 * 8 + 2 * 8,
     "adi_line_vz": 9 * 8 + 2 * 8,

@@ -1725,6 +1725,9 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
     int* cand_cnt = d.cand_cnt;
     int* cand = d.cand;
     unsigned char* dryflag = d.dryflag;
+    /* the run tables are consumed for this half-step (solver and sediment
+     * sweeps); the segment finder at its end appends from zero */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
     /* each lane's candidate list lives in LDS (a private array would sit in
      * scratch memory, one global round trip per sort step) */
     __shared__ int s_list[64][kCand + 1];
@@ -1790,14 +1793,9 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
  * and t_v at (a,b-1),(a,b); each cell applies the flags of itself and of the
  * neighbours that write it, then copies.  Face depths read the new z (t_z)
  * inside [0,N]x[0,M] and the untouched z outside, as after update_uvz. */
-__global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int kd, int kn, int tiled)
+__device__ __forceinline__ void uvz_cell(const HdDevG& d, int i, int j, int kd, int kn)
 {
-    const HdDevG& d = dev_view(dv);
     const unsigned char* dryflag = d.dryflag;
-    /* the run tables are rebuilt by the next kernel on the stream */
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
-    int i, j;
-    if (!row_cell(d, tiled, i, j)) return;
     int N = d.N, M = d.M, W = d.W;
     int p = i * W + j;
     bool me = dryflag[p];
@@ -1820,6 +1818,16 @@ __global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int k
         d.Htdu[p] = (h[i * W + j - 1] + h[p] + zd + zc) * 0.5;
         d.Htdv[p] = (h[(i - 1) * W + j] + h[p] + zr + zc) * 0.5;
     }
+}
+
+__global__ void __launch_bounds__(256) k_uvz(const HdDev* __restrict__ dv, int kd, int kn, int tiled)
+{
+    const HdDevG& d = dev_view(dv);
+    /* the run tables are rebuilt by the next kernel on the stream */
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
+    uvz_cell(d, i, j, kd, kn);
 }
 
 /* After u, v, z were written from outside the step: t_u = u, t_v = v,
@@ -1951,10 +1959,14 @@ __global__ void k_reset_runs(const HdDev* __restrict__ dv)
  * number of runs found. */
 template <bool ROW>
 __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, int SL, int lane, int* dau,
-                                         int* cuoi, int* sf, int* sl, int2 ext)
+                                         int* cuoi, int* sf, int* sl, int2 ext, const unsigned char* dry)
 {
+    /* wet: khouot == 0 and, when the state copy of this half-step runs in
+     * the same launch, not about to be dried by it */
     auto wetv = [&](int x) -> int {
-        return (x >= 2 && x <= L) ? (kh[ROW ? line * W + x : x * W + line] == 0) : 0;
+        if (x < 2 || x > L) return 0;
+        const int q = ROW ? line * W + x : x * W + line;
+        return kh[q] == 0 && !(dry && dry[q]);
     };
     const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
     int ns = 0, ne = 0;
@@ -2004,13 +2016,13 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
  * of its own line, so the cells of one edge are independent.  The velocity
  * loads of all three edges go out together; per edge the FS loads follow
  * (one round trip), with a barrier between the edges. */
-__device__ void fs_edges_block(const HdDevG& d)
+__device__ void fs_edges_block(const HdDevG& d, const unsigned char* dry, int kd, int kn)
 {
     const int N = d.N, M = d.M, W = d.W;
     const int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
     const int T = blockDim.x, tid = threadIdx.x;
     constexpr int K = 8;
-    if (M - 1 > K * T || imax > K * T) { /* very wide grid: edge by edge */
+    if (!dry && (M - 1 > K * T || imax > K * T)) { /* very wide grid: edge by edge */
         for (int j = 1 + tid; j < M; j += T) fs_ng(d, N, N - 1, -1, j);
         __syncthreads();
         for (int j = 1 + tid; j < M; j += T) fs_ng(d, 2, 2, 1, j);
@@ -2031,10 +2043,24 @@ __device__ void fs_edges_block(const HdDevG& d)
             if (p < 2) {
                 const int loc = p == 0 ? N : 2;
                 valid = x < M;
-                if (valid) { tq = d.t_u[loc * W + x]; q = d.u[loc * W + x]; }
+                if (valid) {
+                    tq = d.t_u[loc * W + x];
+                    q = d.u[loc * W + x];
+                    if (dry) { /* the values the state copy leaves (k_uvz) */
+                        if (dry[loc * W + x] || dry[(loc + 1) * W + x]) tq = 0;
+                        q = tq * (1 - kd);
+                    }
+                }
             } else {
                 valid = x <= imax;
-                if (valid) { tq = d.t_v[x * W + M]; q = d.v[x * W + M]; }
+                if (valid) {
+                    tq = d.t_v[x * W + M];
+                    q = d.v[x * W + M];
+                    if (dry) {
+                        if (dry[x * W + M] || dry[x * W + M + 1]) tq = 0;
+                        if (x <= N) q = tq * (1 - kn);
+                    }
+                }
             }
             on[p][b] = valid && tq > 0;
             dl[p][b] = (q + tq) * 0.25 * dT;
@@ -2054,7 +2080,8 @@ __device__ void fs_edges_block(const HdDevG& d)
             if (i > N + 1) i = N + 1;
             const int i_deltaX = hd_rint_int(deltaX);
             const int q = i_dX != 0 ? i_deltaX / i_dX : 0;
-            const double f0 = d.FS[i * W + j], f1 = d.FS[(i + 1) * W + j];
+            double f0 = d.FS[i * W + j], f1 = d.FS[(i + 1) * W + j];
+            if (dry) { f0 = dry[i * W + j] ? 0.0 : f0; f1 = dry[(i + 1) * W + j] ? 0.0 : f1; }
             d.FS[location * W + j] = f0 + sign * (f1 - f0) * (deltaX / dX - q);
         }
         __syncthreads();
@@ -2071,7 +2098,8 @@ __device__ void fs_edges_block(const HdDevG& d)
             if (j > M + 1) j = M + 1;
             const int j_deltaY = hd_rint_int(deltaY);
             const int q = j_dY != 0 ? j_deltaY / j_dY : 0;
-            const double f0 = d.FS[i * W + j], f1 = d.FS[i * W + j + 1];
+            double f0 = d.FS[i * W + j], f1 = d.FS[i * W + j + 1];
+            if (dry) { f0 = dry[i * W + j] ? 0.0 : f0; f1 = dry[i * W + j + 1] ? 0.0 : f1; }
             d.FS[i * W + M] = f1 + (f1 - f0) * (deltaY / dY - q);
         }
     }
@@ -2086,32 +2114,33 @@ __device__ void fs_edges_block(const HdDevG& d)
  * rows 4b .. 4b+3; with count_wet these waves also add the row's
  * khouot == 0 cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
  * [nrowb, nrowb + ncolb): columns 2 + 4b' ... */
-__global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
+__device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int count_wet,
+                                                  const unsigned char* dry, int kd, int kn)
 {
-    const HdDevG& d = dev_view(dv);
-    if (blockIdx.x == (unsigned)(nrowb + ncolb)) { /* the FS edge interpolations (independent of the mask) */
-        fs_edges_block(d);
+    if (bid == nrowb + ncolb) { /* the FS edge interpolations (independent of the mask) */
+        fs_edges_block(d, dry, kd, kn);
         return;
     }
     __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
     const int N = d.N, M = d.M, W = d.W, SL = d.SL;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int* kh = d.khouot;
-    const bool rows = blockIdx.x < (unsigned)nrowb;
+    const bool rows = bid < nrowb;
     int n = 0, line;
     if (rows) {
-        const int i = 4 * blockIdx.x + wv; /* rows 0..N+2 for counting, 2..N for runs */
+        const int i = 4 * bid + wv; /* rows 0..N+2 for counting, 2..N for runs */
         line = i;
         if (count_wet && i <= N + 2) {
             int c = 0;
             const int2 e = d.ext[i];
             const int hi = e.y < M + 2 ? e.y : M + 2;
-            for (int j = e.x + lane; j <= hi; j += kWave) c += (kh[i * W + j] == 0);
+            for (int j = e.x + lane; j <= hi; j += kWave) c += (kh[i * W + j] == 0 && !(dry && dry[i * W + j]));
             for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
             if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
         }
         if (i >= 2 && i <= N) {
-            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv], d.ext[i]);
+            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv], d.ext[i],
+                                     dry);
             n = ns < SL ? ns : SL;
             if (lane == 0) {
                 d.moci[i] = n;
@@ -2119,11 +2148,11 @@ __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ d
             }
         }
     } else {
-        const int j = 2 + 4 * (blockIdx.x - nrowb) + wv;
+        const int j = 2 + 4 * (bid - nrowb) + wv;
         line = j;
         if (j <= M) {
             int ns = scan_runs<false>(kh, W, j, N, SL, lane, d.dauj + j * SL, d.cuoij + j * SL, s_f[wv], s_l[wv],
-                                      d.ext[N + 3 + j]);
+                                      d.ext[N + 3 + j], dry);
             n = ns < SL ? ns : SL;
             if (lane == 0) {
                 d.mocj[j] = n;
@@ -2145,6 +2174,44 @@ __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ d
     if (lane < n && at < cap) table[at] = make_int4(line, s_f[wv][lane], s_l[wv][lane], 0);
 }
 
+
+__global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
+{
+    find_limits_block(dev_view(dv), blockIdx.x, nrowb, ncolb, count_wet, nullptr, 0, 0);
+}
+
+/* The state copy (k_uvz) and the segment tables + FS edges (k_find_limits)
+ * of one half-step in one launch: blocks [0, nrowb + ncolb] build the
+ * tables, the rest copy the state over the active row tiles (or the whole
+ * grid).  The table blocks do not wait for the copy: a cell is wet iff
+ * khouot == 0 and its drying flag is clear (the copy sets khouot = 1 exactly
+ * at flagged cells), and the FS edge block applies the copy's zeroing and
+ * u = t_u (1 - kd), v = t_v (1 - kn) to what it reads, so either order of
+ * the two halves gives the result of copying first.  The run counters were
+ * cleared by the smoothing pass before the wetting kernel. */
+__global__ void __launch_bounds__(256) k_uvz_segments(const HdDev* __restrict__ dv, int kd, int kn, int tiled,
+                                                       int nrowb, int ncolb, int count_wet)
+{
+    const HdDevG& d = dev_view(dv);
+    const int nseg = nrowb + ncolb + 1;
+    if ((int)blockIdx.x < nseg) {
+        find_limits_block(d, blockIdx.x, nrowb, ncolb, count_wet, d.dryflag, kd, kn);
+        return;
+    }
+    const int t = ((int)blockIdx.x - nseg) * 4 + (threadIdx.x >> 6);
+    const int ncb = (d.M + 3 + 63) >> 6;
+    int tile;
+    if (tiled) {
+        if (t >= d.ntiles[0]) return;
+        tile = d.tiles[t];
+    } else {
+        if (t >= (d.N + 3) * ncb) return;
+        tile = t;
+    }
+    const int i = tile / ncb, j = (tile % ncb) * 64 + (threadIdx.x & 63);
+    if (j >= d.M + 3) return;
+    uvz_cell(d, i, j, kd, kn);
+}
 
 __global__ void k_wet_count(const HdDev* __restrict__ dv)
 {
@@ -2211,6 +2278,22 @@ void hd_launch_find_limits(const HdBatch& b, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reset_runs, dim3(1, 1, b.n), dim3(64), 0, s, b.dv);
     hd_launch_segments(b, 0, s);
+}
+
+bool hd_uvz_segments_fits(const HdBatch& b)
+{
+    /* the fused kernel's FS edge block holds up to 8 cells per thread */
+    const int imax = (b.M - 1 < b.N + 2) ? b.M - 1 : b.N + 2;
+    return b.M - 1 <= 8 * 256 && imax <= 8 * 256;
+}
+
+void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s)
+{
+    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + 3) / 4;
+    int ncb = (b.M + 3 + 63) / 64;
+    int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ncb;
+    hipLaunchKernelGGL(k_uvz_segments, dim3(nrowb + ncolb + 1 + (tiles + 3) / 4, 1, b.n), dim3(256), 0, s, b.dv, kd,
+                       kn, ntiles > 0 ? 1 : 0, nrowb, ncolb, count);
 }
 
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s)
