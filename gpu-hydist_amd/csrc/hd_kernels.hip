@@ -1966,7 +1966,9 @@ __device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, 
     auto wetv = [&](int x) -> int {
         if (x < 2 || x > L) return 0;
         const int q = ROW ? line * W + x : x * W + line;
-        return kh[q] == 0 && !(dry && dry[q]);
+        const int k = kh[q];
+        const int f = dry ? dry[q] : 0; /* loaded with khouot, not after it */
+        return k == 0 && !f;
     };
     const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
     int ns = 0, ne = 0;
@@ -2105,15 +2107,110 @@ __device__ void fs_edges_block(const HdDevG& d, const unsigned char* dry, int kd
     }
 }
 
+/* The column tables of Find_Calculation_limits_Vertical (the run rule of
+ * scan_runs, support_funcs.cu:252-279) for kColG adjacent columns per block
+ * of 256 threads: thread (chunk c, column q) turns 32 rows of its column into
+ * a wet bit mask -- a wave's loads are 8 adjacent columns of 8 rows, instead
+ * of 64 rows of one column, all issued before any is used -- the
+ * neighbouring chunks' edge bits come through LDS, starts and ends are bit
+ * operations on the masks and their ranks prefix sums over the chunks.  Rows
+ * beyond 1024 are done in further passes with the counts and the last bit
+ * carried. */
+constexpr int kColG = 8;                 /* columns per block */
+constexpr int kColCh = 256 / kColG;      /* chunks of 32 rows per pass */
+
+__device__ __forceinline__ int col_group_scan(const HdDevG& d, int g, const unsigned char* dry, int* s_f, int* s_l)
+{
+    const int N = d.N, M = d.M, W = d.W, SL = d.SL, L = N;
+    const int q = threadIdx.x % kColG, c = threadIdx.x / kColG;
+    const int j = 2 + kColG * g + q;
+    const bool colok = j <= M;
+    const int jc = colok ? j : M; /* a column inside the grid for the loads of idle threads */
+    const int* kh = d.khouot;
+    __shared__ unsigned int s_m[kColCh][kColG];
+    __shared__ int s_cs[kColCh][kColG], s_ce[kColCh][kColG];
+    __shared__ int s_ts[kColG], s_te[kColG], s_pb[kColG];
+    auto wet = [&](int x) -> int {
+        if (!colok || x < 2 || x > L) return 0;
+        const int p = x * W + j;
+        const int k = kh[p];
+        const int f = dry ? dry[p] : 0;
+        return k == 0 && !f;
+    };
+    /* rows 2 .. L in passes of 1024 (the loads of a pass do not wait for any
+     * other value); the column's wetness at L-2 and L-1 for its end rule */
+    const int wl2 = wet(L - 2), wl1 = wet(L - 1);
+    if (c == 0) { s_ts[q] = 0; s_te[q] = 0; s_pb[q] = 0; }
+    const int xhi = L;
+    for (int x0 = 2; x0 <= xhi; x0 += 32 * kColCh) {
+        const int xs = x0 + 32 * c;
+        int kv[32];
+        unsigned char dv_[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int x = xs + r <= L ? xs + r : L;
+            kv[r] = kh[x * W + jc];
+            dv_[r] = dry ? dry[x * W + jc] : 0;
+        }
+        unsigned int m = 0;
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int x = xs + r;
+            const bool w = colok && x <= xhi && kv[r] == 0 && !dv_[r];
+            m |= (unsigned int)w << r;
+        }
+        s_m[c][q] = m;
+        __syncthreads();
+        const unsigned int prev = c > 0 ? (s_m[c - 1][q] >> 31) : (unsigned int)s_pb[q];
+        const unsigned int next = c < kColCh - 1 ? (s_m[c + 1][q] & 1u) : (unsigned int)wet(x0 + 32 * kColCh);
+        const int nle = L - 2 - xs + 1; /* rows of the chunk at or before L-2 */
+        const unsigned int le2 = nle >= 32 ? ~0u : (nle <= 0 ? 0u : ((1u << nle) - 1));
+        unsigned int extra = 0;
+        if (wl2 && L - 1 >= xs && L - 1 < xs + 32) extra |= 1u << (L - 1 - xs);
+        if (wl2 && wl1 && L >= xs && L < xs + 32) extra |= 1u << (L - xs);
+        const unsigned int st = m & ~((m << 1) | prev) & le2;
+        const unsigned int en = m & ~((m >> 1) | (next << 31)) & (le2 | extra);
+        s_cs[c][q] = __popc(st);
+        s_ce[c][q] = __popc(en);
+        __syncthreads();
+        int ks = s_ts[q], ke = s_te[q];
+        for (int k = 0; k < c; ++k) { ks += s_cs[k][q]; ke += s_ce[k][q]; }
+        for (unsigned int b = st; b; b &= b - 1, ++ks) {
+            const int x = xs + __ffs((int)b) - 1;
+            if (ks < SL) { d.dauj[j * SL + ks] = x; s_f[q * 24 + ks] = x; }
+        }
+        for (unsigned int b = en; b; b &= b - 1, ++ke) {
+            const int x = xs + __ffs((int)b) - 1;
+            if (ke < SL) { d.cuoij[j * SL + ke] = x; s_l[q * 24 + ke] = x; }
+        }
+        __syncthreads();
+        if (c == kColCh - 1) {
+            s_ts[q] = ks;
+            s_te[q] = ke;
+            s_pb[q] = (int)(m >> 31);
+        }
+        __syncthreads();
+    }
+    const int ns = s_ts[q];
+    const int n = colok ? (ns < SL ? ns : SL) : 0;
+    if (c == 0 && colok) {
+        d.mocj[j] = n;
+        if (ns > SL) d.flags[0] = 1;
+    }
+    return n;
+}
+
 /* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335):
  * the per-line tables daui / cuoii / moci (rows i = 2..N) and dauj / cuoij /
  * mocj (columns j = 2..M), plus the packed run tables the solvers read
- * (runs_v for rows, runs_u for columns; their counters were zeroed by the
- * previous kernel on the stream).  Blocks of four waves, one line per wave;
- * each block reserves its table slots with one atomic.  Blocks [0, nrowb):
- * rows 4b .. 4b+3; with count_wet these waves also add the row's
- * khouot == 0 cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
- * [nrowb, nrowb + ncolb): columns 2 + 4b' ... */
+ * (runs_v for rows, runs_u for columns; their counters were zeroed earlier
+ * in the half-step).  Blocks of 256 threads; each reserves its table slots
+ * with one atomic.  Blocks [0, nrowb): rows 4b .. 4b+3, one wave per row;
+ * with count_wet these waves also add the row's wet cells (rows 0..N+2) to
+ * the wet-cell accumulator.  Blocks [nrowb, nrowb + ncolb): columns
+ * 2 + 8b' .. 2 + 8b' + 7 (col_group_scan).  Block nrowb + ncolb: the FS
+ * edge interpolations.  `dry` (the drying flags, when the state copy runs in
+ * the same launch) marks cells that count as dry already. */
 __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int count_wet,
                                                   const unsigned char* dry, int kd, int kn)
 {
@@ -2121,11 +2218,33 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
         fs_edges_block(d, dry, kd, kn);
         return;
     }
-    __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
     const int N = d.N, M = d.M, W = d.W, SL = d.SL;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int* kh = d.khouot;
     const bool rows = bid < nrowb;
+    if (!rows) { /* kColG columns per block */
+        __shared__ int s_cf[kColG * 24], s_cl[kColG * 24], s_cn[kColG], s_cbase[kColG];
+        const int g = bid - nrowb;
+        const int n = col_group_scan(d, g, dry, s_cf, s_cl);
+        const int q = threadIdx.x % kColG, c = threadIdx.x / kColG;
+        if (c == 0) s_cn[q] = n;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int k = 0; k < kColG; ++k) tot += s_cn[k];
+            int base = tot ? atomicAdd(d.run_cnt + 0, tot) : 0;
+            for (int k = 0; k < kColG; ++k) { s_cbase[k] = base; base += s_cn[k]; }
+        }
+        __syncthreads();
+        const int cap = (M + 3) * SL;
+        const int j = 2 + kColG * g + q;
+        for (int k = c; k < n; k += kColCh) {
+            const int at = s_cbase[q] + k;
+            if (at < cap) d.runs_u[at] = make_int4(j, s_cf[q * 24 + k], s_cl[q * 24 + k], 0);
+        }
+        return;
+    }
+    __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
     int n = 0, line;
     if (rows) {
         const int i = 4 * bid + wv; /* rows 0..N+2 for counting, 2..N for runs */
@@ -2134,7 +2253,11 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
             int c = 0;
             const int2 e = d.ext[i];
             const int hi = e.y < M + 2 ? e.y : M + 2;
-            for (int j = e.x + lane; j <= hi; j += kWave) c += (kh[i * W + j] == 0 && !(dry && dry[i * W + j]));
+            for (int j = e.x + lane; j <= hi; j += kWave) {
+                const int k = kh[i * W + j];
+                const int f = dry ? dry[i * W + j] : 0;
+                c += (k == 0 && !f);
+            }
             for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
             if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
         }
@@ -2147,29 +2270,17 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
                 if (ns > SL) d.flags[0] = 1;
             }
         }
-    } else {
-        const int j = 2 + 4 * (bid - nrowb) + wv;
-        line = j;
-        if (j <= M) {
-            int ns = scan_runs<false>(kh, W, j, N, SL, lane, d.dauj + j * SL, d.cuoij + j * SL, s_f[wv], s_l[wv],
-                                      d.ext[N + 3 + j], dry);
-            n = ns < SL ? ns : SL;
-            if (lane == 0) {
-                d.mocj[j] = n;
-                if (ns > SL) d.flags[0] = 1;
-            }
-        }
     }
     if (lane == 0) s_n[wv] = n;
     __syncthreads();
     if (threadIdx.x == 0) {
         int tot = s_n[0] + s_n[1] + s_n[2] + s_n[3];
-        int base = tot ? atomicAdd(d.run_cnt + (rows ? 1 : 0), tot) : 0;
+        int base = tot ? atomicAdd(d.run_cnt + 1, tot) : 0;
         for (int w = 0; w < 4; ++w) { s_base[w] = base; base += s_n[w]; }
     }
     __syncthreads();
-    const int cap = rows ? (N + 3) * SL : (M + 3) * SL;
-    int4* table = rows ? d.runs_v : d.runs_u;
+    const int cap = (N + 3) * SL;
+    int4* table = d.runs_v;
     const int at = s_base[wv] + lane;
     if (lane < n && at < cap) table[at] = make_int4(line, s_f[wv][lane], s_l[wv][lane], 0);
 }
@@ -2289,7 +2400,7 @@ bool hd_uvz_segments_fits(const HdBatch& b)
 
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s)
 {
-    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + 3) / 4;
+    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + kColG - 1) / kColG; /* rows: 4 per block; columns: kColG */
     int ncb = (b.M + 3 + 63) / 64;
     int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ncb;
     hipLaunchKernelGGL(k_uvz_segments, dim3(nrowb + ncolb + 1 + (tiles + 3) / 4, 1, b.n), dim3(256), 0, s, b.dv, kd,
@@ -2298,7 +2409,7 @@ void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int co
 
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s)
 {
-    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + 3) / 4;
+    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + kColG - 1) / kColG; /* rows: 4 per block; columns: kColG */
     /* + one block for the FS edge interpolations */
     hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb + 1, 1, b.n), dim3(256), 0, s, b.dv, nrowb, ncolb, count);
 }
