@@ -171,6 +171,23 @@ def test_full_river_bitwise():
     assert_same_state(o, g, "river 980x1048, 4 steps")
 
 
+def test_full_river_sediment_bitwise():
+    """The 980 x 1048 river with suspended sediment from t = 0 and the bed
+    updated every step, 120 steps (10 simulated minutes), against the
+    checker: speculative Thomas in both sweeps, the fused state copy and
+    tables, FS edges and bed change at full size."""
+    oracle.set_threads(min(16, len(os.sched_getaffinity(0))))
+    case = synth.config(3)
+    case.sediment_start = 0.0
+    case.bed_change_start = 0.0
+    case.bed_mode = BED_EVERY_STEP
+    o, g = pair(case)
+    op = case.opts()
+    o.run(0.0, 120, op)
+    g.run(0.0, 120, op)
+    assert_same_state(o, g, "river 980x1048 with sediment and bed change, 120 steps")
+
+
 def full_river():
     return synth.config(2)
 
