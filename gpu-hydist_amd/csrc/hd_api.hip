@@ -635,17 +635,21 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else {
             launch(x, "adi_spec_vz", [&] { hd_launch_adi_spec(x.b(), 0, shift, x.run_blocks_v, s); });
         }
-        launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
-        x.swap_v();
         if (x.solver == 0) {
-            /* columns: the cells by row tiles (coalesced), then the margins per line */
-            launch(x, "explicit_rows_u", [&] { hd_launch_explicit_rows_u(x.b(), x.bw, ntiles, s); });
+            /* smoothing of t_v and the cells of solveU by row tiles in one
+             * launch, then the margins per column */
+            launch(x, "smooth_v_explicit_u", [&] { hd_launch_smooth_v_explicit_u(x.b(), x.bw, ntiles, s); });
+            x.swap_v();
             launch(x, "margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 0, s); });
-        } else if (x.solver == 3) {
-            launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 1, s); });
         } else {
-            launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
-            launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
+            launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
+            x.swap_v();
+            if (x.solver == 3) {
+                launch(x, "explicit_margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 1, s); });
+            } else {
+                launch(x, "explicit_u", [&] { hd_launch_explicit(x.b(), 1, s); });
+                launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
+            }
         }
         launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, ntiles, s); });
         x.swap_u();

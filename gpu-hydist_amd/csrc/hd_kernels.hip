@@ -1711,6 +1711,58 @@ __global__ void __launch_bounds__(256) k_explicit_rows_u(const HdDev* __restrict
     if (in) d.t_u[p] = u_solver(d, i, j);
 }
 
+/* Normalize of t_v (support_funcs.cu:682-703, the V form: neighbours along
+ * the row always, across rows when wet) at cell (qi, qj), from the unsmoothed
+ * buffer, with every load issued unconditionally.  The same operations as
+ * k_normalize(isU = 0), so the value equals what that pass writes. */
+__device__ __forceinline__ double smooth_v_at(const HdDevG& d, int qi, int qj)
+{
+    const int N = d.N, M = d.M, W = d.W;
+    const int q = qi * W + qj;
+    const bool in = qi >= 2 && qi <= N && qj >= 2 && qj <= M;
+    const double* val = d.t_v;
+    const double r0 = val[q];
+    const double vcm = val[in ? q - 1 : q], vcp = val[in ? q + 1 : q];
+    const double vfm = val[in ? q - W : q], vfp = val[in ? q + W : q];
+    const int kfm = d.khouot[in ? q - W : q], kfp = d.khouot[in ? q + W : q];
+    double r = r0;
+    if (in && r != 0) {
+        int count = 2;
+        double nb = vcm + vcp;
+        if (kfm == 0) { nb += vfm; count++; }
+        if (kfp == 0) { nb += vfp; count++; }
+        r = r * d.P.heso + (1 - d.P.heso) * nb / count;
+    }
+    return r;
+}
+
+/* The VZ half-step's smoothing of t_v (Normalize + update_buffer into the
+ * partner buffer, as k_normalize) and solveU (k_explicit_rows_u) in one pass
+ * over the active row tiles: solveU reads the smoothed t_v at the four faces
+ * around its cell, which this thread recomputes from the unsmoothed buffer
+ * (the values the smoothing pass writes), so the two need no launch
+ * boundary between them. */
+__global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled)
+{
+    const HdDevG& d = dev_view(dv);
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
+    const int N = d.N, M = d.M, W = d.W;
+    const int p = i * W + j;
+    const double sm = smooth_v_at(d, i, j);
+    d.tmp_v[p] = sm;
+    if (j < 2 || j > M || i < 2 || i > N - 1) return;
+    const int* kh = d.khouot;
+    const bool in = kh[p] == 0 && kh[p + W] == 0 && (i <= N - 2 || kh[(N - 2) * W + j] == 0);
+    if (!in) return;
+    ExRaw r = u_solver_ld(d, i, j);
+    r.t0 = smooth_v_at(d, i, j - 1);
+    r.t1 = sm;
+    r.t2 = smooth_v_at(d, i + 1, j - 1);
+    r.t3 = smooth_v_at(d, i + 1, j);
+    d.t_u[p] = u_solver_ev(d.P, r);
+}
+
 /* ================================================================ wet / dry */
 
 /* Wetting pass of Reset_states_horizontal (support_funcs.cu:92-112) /
@@ -2487,6 +2539,11 @@ void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_
     int nl = (isU ? b.M : b.N) - 1;
     if (isU) hipLaunchKernelGGL(k_explicit_margins<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
     else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
+}
+
+void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_smooth_v_explicit_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
 }
 
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
