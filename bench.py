@@ -47,6 +47,7 @@ KERNEL_BYTES = {
     # smoothing of t_v with explicit u in one pass: t_v, khouot read, the
     # partner written, plus explicit u's 80 B
     "smooth_v_explicit_u": 8 + 4 + 8 + 9 * 8 + 8,
+    "smooth_u_explicit_v": 8 + 4 + 8 + 9 * 8 + 8,
     # state copy + drying + face depths: t_z, t_u, t_v, H_moi, h read;
     # z, u, v, Htdu, Htdv written
     "uvz_dry_htd": 5 * 8 + 5 * 8,

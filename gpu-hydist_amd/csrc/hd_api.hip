@@ -610,14 +610,21 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else {
             launch(x, "adi_spec_uz", [&] { hd_launch_adi_spec(x.b(), 1, shift, x.run_blocks_u, s); });
         }
-        launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
-        x.swap_u();
-        if (x.solver == 0 || x.solver == 3) {
-            /* rows are coalesced already: one wave per line, margins after the cells */
-            launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, 1, s); });
+        if (x.solver == 0) {
+            /* smoothing of t_u and the cells of solveV by row tiles in one
+             * launch, then the margins per row */
+            launch(x, "smooth_u_explicit_v", [&] { hd_launch_smooth_u_explicit_v(x.b(), x.bw, ntiles, s); });
+            x.swap_u();
+            launch(x, "margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, 0, s); });
         } else {
-            launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
-            launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
+            launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
+            x.swap_u();
+            if (x.solver == 3) {
+                launch(x, "explicit_margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, 1, s); });
+            } else {
+                launch(x, "explicit_v", [&] { hd_launch_explicit(x.b(), 0, s); });
+                launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
+            }
         }
         launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, ntiles, s); });
         x.swap_v();

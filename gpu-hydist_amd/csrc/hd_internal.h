@@ -116,6 +116,7 @@ void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_t s);
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
+void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
 void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);

@@ -1736,6 +1736,56 @@ __device__ __forceinline__ double smooth_v_at(const HdDevG& d, int qi, int qj)
     return r;
 }
 
+/* Normalize of t_u (the U form: neighbours along the column always, across
+ * columns when wet) at cell (qi, qj), as smooth_v_at. */
+__device__ __forceinline__ double smooth_u_at(const HdDevG& d, int qi, int qj)
+{
+    const int N = d.N, M = d.M, W = d.W;
+    const int q = qi * W + qj;
+    const bool in = qi >= 2 && qi <= N && qj >= 2 && qj <= M;
+    const double* val = d.t_u;
+    const double r0 = val[q];
+    const double vcm = val[in ? q - W : q], vcp = val[in ? q + W : q];
+    const double vfm = val[in ? q - 1 : q], vfp = val[in ? q + 1 : q];
+    const int kfm = d.khouot[in ? q - 1 : q], kfp = d.khouot[in ? q + 1 : q];
+    double r = r0;
+    if (in && r != 0) {
+        int count = 2;
+        double nb = vcm + vcp;
+        if (kfm == 0) { nb += vfm; count++; }
+        if (kfp == 0) { nb += vfp; count++; }
+        r = r * d.P.heso + (1 - d.P.heso) * nb / count;
+    }
+    return r;
+}
+
+/* The UZ half-step's smoothing of t_u and solveV (UVZSolver_multithread.cu:
+ * 1055-1071) in one pass over the active row tiles: the cells first ..
+ * last-1 of every wet run of every row, recognised from the mask as in
+ * k_explicit_rows_u (k and k+1 wet, and for k = M-1 also M-2), with the
+ * smoothed t_u at the four faces around the cell recomputed from the
+ * unsmoothed buffer.  k_explicit_margins (cells off) writes the margins. */
+__global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled)
+{
+    const HdDevG& d = dev_view(dv);
+    int i, j;
+    if (!row_cell(d, tiled, i, j)) return;
+    const int N = d.N, M = d.M, W = d.W;
+    const int p = i * W + j;
+    const double sm = smooth_u_at(d, i, j);
+    d.tmp_u[p] = sm;
+    if (i < 2 || i > N || j < 2 || j > M - 1) return;
+    const int* kh = d.khouot;
+    const bool in = kh[p] == 0 && kh[p + 1] == 0 && (j <= M - 2 || kh[i * W + M - 2] == 0);
+    if (!in) return;
+    ExRaw r = v_solver_ld(d, i, j);
+    r.t0 = smooth_u_at(d, i - 1, j);
+    r.t1 = sm;
+    r.t2 = smooth_u_at(d, i - 1, j + 1);
+    r.t3 = smooth_u_at(d, i, j + 1);
+    d.t_v[p] = v_solver_ev(d.P, r);
+}
+
 /* The VZ half-step's smoothing of t_v (Normalize + update_buffer into the
  * partner buffer, as k_normalize) and solveU (k_explicit_rows_u) in one pass
  * over the active row tiles: solveU reads the smoothed t_v at the four faces
@@ -2539,6 +2589,11 @@ void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_
     int nl = (isU ? b.M : b.N) - 1;
     if (isU) hipLaunchKernelGGL(k_explicit_margins<true>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
     else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
+}
+
+void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_smooth_u_explicit_v, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
 }
 
 void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
