@@ -37,9 +37,9 @@ constexpr int kSpecWave = 64;
  * shortens the critical path from the run's length to about kSpecWF + L rows
  * (L = rows per lane).  Back substitution, x_i = bp_i + ap_i x_{i+1}, is an
  * affine chain and is treated the same way with 2-entry affine maps. */
-constexpr int kSpecWF = 24;           /* forward warm-up rows after the estimate */
+constexpr int kSpecWF = 16;           /* forward warm-up rows after the estimate */
 constexpr int kSpecWB = 32;           /* back-substitution warm-up rows */
-constexpr int kSpecSerialF = 48;      /* up to this many rows the forward sweep stays on one lane */
+constexpr int kSpecSerialF = 32;      /* up to this many rows the forward sweep stays on one lane */
 constexpr int kSpecSerialB = 96;      /* likewise for back substitution */
 constexpr int kSpecMaxL = 8;          /* rows per lane of one speculative sweep (chunks <= 512 rows) */
 
