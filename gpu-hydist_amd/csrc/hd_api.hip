@@ -60,12 +60,15 @@ struct ArrMirror {
 struct GraphKey {
     int sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode;
     double s_start;
-    int ntiles;   /* per-row kernels over this many active tiles (0: full grid) */
+    int ntiles;       /* per-row kernels over this many active tiles (0: full grid) */
+    int sed_ntiles;   /* the same for the sediment sweep's pointwise kernels */
+    int bed_ntiles;   /* and for the bed-load flux */
     bool operator<(const GraphKey& o) const
     {
-        return std::tie(sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode, s_start, ntiles) <
+        return std::tie(sed0, sed1, bed, count, kd, kn, analytic, ketdinh, qb_mode, s_start, ntiles, sed_ntiles,
+                        bed_ntiles) <
                std::tie(o.sed0, o.sed1, o.bed, o.count, o.kd, o.kn, o.analytic, o.ketdinh, o.qb_mode, o.s_start,
-                        o.ntiles);
+                        o.ntiles, o.sed_ntiles, o.bed_ntiles);
     }
 };
 
@@ -113,6 +116,8 @@ struct hd_ctx {
     double* z_snap = nullptr;   /* z snapshot for hd_z_change (allocated on first use), + 3 result slots */
     int ntiles = 0;             /* active row tiles (computed at init / prepare) */
     bool settled = false;       /* a full step has run since init / upload: land tiles are at a fixed point */
+    bool sed_settled = false;   /* ... and a sediment sweep over the whole grid (VTH, hesoK, FS = tFS) */
+    bool bed_settled = false;   /* ... and a bed-load flux over the whole grid */
     int rb_mult = 2;            /* solver blocks per line (autotuned) */
     int bw = 256;               /* block width of the per-row full-grid kernels (autotuned) */
     Prof prof;
@@ -450,6 +455,8 @@ int hd_upload(hd_ctx* c, const char* name, const void* src, size_t nbytes)
     HIPCHK(c, hipMemcpyAsync(p, src, nb, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->settled = false;
+    c->sed_settled = false;
+    c->bed_settled = false;
     /* a new mask can open cells outside the active tiles */
     if (c->initialised && std::strcmp(name, "khouot") == 0) return compute_tiles(c);
     return HD_OK;
@@ -522,6 +529,8 @@ static int compute_tiles(hd_ctx* c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->ntiles = n;
     c->settled = false;
+    c->sed_settled = false;
+    c->bed_settled = false;
     return HD_OK;
 }
 
@@ -591,7 +600,7 @@ static int check_series(hd_ctx* c, double t, const hd_step_opts& o)
  * time is advanced by the boundary kernel.  Returns in *slo / *eoff the row
  * range of a following bed change (rows slo .. N - eoff - 1). */
 static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sediment, int count_wet, int ntiles,
-                         int* slo_out, int* eoff_out)
+                         int sed_ntiles, int* slo_out, int* eoff_out)
 {
     hipStream_t s = x.stream;
     bool channel = o.kenhhepd || o.kenhhepng;
@@ -669,7 +678,7 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         launch(x, "segments_fs_edges", [&] { hd_launch_segments(x.b(), count_wet, s); });
     }
     if (sediment) {
-        launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), s); });
+        launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), sed_ntiles, s); });
         if (x.solver_v1) {
             launch(x, which == 0 ? "fs_coef_j" : "fs_coef_i",
                    [&] { hd_launch_fs_coef(x.b(), which, o.bed_change_start, o.ketdinh, s); });
@@ -683,25 +692,25 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
                                  which == 0 ? x.run_blocks_u : x.run_blocks_v, x.solver == 0, s);
             });
         }
-        launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), s); });
+        launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), sed_ntiles, s); });
         if (which == 1) { slo = 3; eoff = 1; }
     }
     if (slo_out) *slo_out = slo;
     if (eoff_out) *eoff_out = eoff;
 }
 
-static void enqueue_bed(Exec& x, const hd_step_opts& o, int slo, int eoff)
+static void enqueue_bed(Exec& x, const hd_step_opts& o, int slo, int eoff, int bed_ntiles)
 {
-    launch(x, "calc_qb", [&] { hd_launch_calc_qb(x.b(), o.qb_mode, x.stream); });
-    launch(x, "bedload", [&] { hd_launch_bedload(x.b(), o.ketdinh, slo, eoff, x.stream); });
+    launch(x, "calc_qb", [&] { hd_launch_calc_qb(x.b(), o.qb_mode, bed_ntiles, x.stream); });
+    launch(x, "bedload", [&] { hd_launch_bedload(x.b(), o.ketdinh, slo, eoff, bed_ntiles, x.stream); });
 }
 
 static void enqueue_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
 {
     int slo = 2, eoff = 0;
-    enqueue_half(x, 0, o, k.sed0, 0, k.ntiles, nullptr, nullptr);
-    enqueue_half(x, 1, o, k.sed1, k.count, k.ntiles, &slo, &eoff);
-    if (k.bed) enqueue_bed(x, o, slo, eoff);
+    enqueue_half(x, 0, o, k.sed0, 0, k.ntiles, k.sed_ntiles, nullptr, nullptr);
+    enqueue_half(x, 1, o, k.sed1, k.count, k.ntiles, k.sed_ntiles, &slo, &eoff);
+    if (k.bed) enqueue_bed(x, o, slo, eoff, k.bed_ntiles);
 }
 
 static GraphKey make_key(hd_ctx* c, double t, const hd_step_opts& o, int count)
@@ -718,6 +727,12 @@ static GraphKey make_key(hd_ctx* c, double t, const hd_step_opts& o, int count)
     k.ketdinh = o.ketdinh; k.qb_mode = o.qb_mode;
     k.s_start = o.bed_change_start;
     k.ntiles = c->settled ? c->ntiles : 0;
+    /* the first step that runs the sediment sweep (the bed-load flux) does
+     * it over the whole grid; later ones over the active tiles */
+    k.sed_ntiles = (c->settled && c->sed_settled) ? c->ntiles : 0;
+    k.bed_ntiles = (c->settled && c->bed_settled) ? c->ntiles : 0;
+    if (k.sed0 || k.sed1) c->sed_settled = true;
+    if (k.bed) c->bed_settled = true;
     return k;
 }
 
@@ -753,7 +768,7 @@ int hd_halfstep(hd_ctx* c, int which, double t, const hd_step_opts* o)
     if (rc) return rc;
     Exec x = exec_of(c);
     set_time(x, t - c->P.dT * 0.5);
-    enqueue_half(x, which, *o, t >= o->sediment_start, 0, 0, nullptr, nullptr);
+    enqueue_half(x, which, *o, t >= o->sediment_start, 0, 0, 0, nullptr, nullptr);
     HIPCHK(c, hipGetLastError());
     return HD_OK;
 }
@@ -765,7 +780,7 @@ int hd_bed_change(hd_ctx* c, double t, const hd_step_opts* o)
     if ((o->kenhhepd || o->kenhhepng) && o->kenhhepng == 1) slo = 3;
     if (t >= o->sediment_start) { slo = 3; eoff = 1; }
     Exec x = exec_of(c);
-    enqueue_bed(x, *o, slo, eoff);
+    enqueue_bed(x, *o, slo, eoff, 0);
     HIPCHK(c, hipGetLastError());
     return HD_OK;
 }
@@ -827,16 +842,16 @@ static void enqueue_sed_step(Exec& x, const hd_step_opts& o, const GraphKey& k)
     for (int which = 0; which < 2; ++which) {
         launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, x.stream); });
         if (which == 0 ? k.sed0 : k.sed1) {
-            launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), x.stream); });
+            launch(x, "vth_hesok", [&] { hd_launch_vth_hesok(x.b(), k.sed_ntiles, x.stream); });
             launch(x, which == 0 ? (x.solver == 0 ? "fs_spec_j" : "fs_run_j") : (x.solver == 0 ? "fs_spec_i" : "fs_run_i"), [&] {
                 hd_launch_fs_run(x.b(), which, o.bed_change_start, o.ketdinh,
                                  which == 0 ? x.run_blocks_u : x.run_blocks_v, x.solver == 0, x.stream);
             });
-            launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), x.stream); });
+            launch(x, "update_fs", [&] { hd_launch_update_fs(x.b(), k.sed_ntiles, x.stream); });
             if (which == 1) { slo = 3; eoff = 1; }
         }
     }
-    if (k.bed) enqueue_bed(x, o, slo, eoff);
+    if (k.bed) enqueue_bed(x, o, slo, eoff, k.bed_ntiles);
 }
 
 static int run_impl(hd_ctx* c, double t, int nsteps, const hd_step_opts* o, double* t_out, int count)
@@ -968,6 +983,8 @@ int hd_flow_refresh(hd_ctx* c)
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->settled = false;
+    c->sed_settled = false;
+    c->bed_settled = false;
     return HD_OK;
 }
 
@@ -989,6 +1006,8 @@ int hd_copy_state(hd_ctx* dst, hd_ctx* src)
     HIPCHK(dst, hipStreamSynchronize(dst->stream));
     dst->ntiles = src->ntiles;
     dst->settled = src->settled;
+    dst->sed_settled = src->sed_settled;
+    dst->bed_settled = src->bed_settled;
     dst->initialised = src->initialised;
     return HD_OK;
 }
@@ -1013,6 +1032,8 @@ int hd_clone(hd_ctx* src, hd_ctx** out)
     set_knobs(c, src->rb_mult, src->bw);
     c->ntiles = src->ntiles;
     c->settled = src->settled;
+    c->sed_settled = src->sed_settled;
+    c->bed_settled = src->bed_settled;
     c->initialised = true;
     return HD_OK;
 }
@@ -1304,15 +1325,27 @@ int hd_ensemble_run(hd_ensemble* e, double t, int nsteps, const hd_step_opts* o,
            e->run_blocks_v, c0->bw, &e->prof, nullptr};
     set_time(x, t);
     std::string why;
-    bool settled = true;
+    bool settled = true, sed_settled = true, bed_settled = true;
     int maxtiles = 0;
     for (hd_ctx* c : e->m) {
         settled = settled && c->settled;
+        sed_settled = sed_settled && c->sed_settled;
+        bed_settled = bed_settled && c->bed_settled;
         maxtiles = std::max(maxtiles, c->ntiles);
     }
     auto key = [&](double tt) {
         GraphKey k = make_key(c0, tt, *o, count);
         k.ntiles = settled ? maxtiles : 0;
+        k.sed_ntiles = (settled && sed_settled) ? maxtiles : 0;
+        k.bed_ntiles = (settled && bed_settled) ? maxtiles : 0;
+        if (k.sed0 || k.sed1) {
+            sed_settled = true;
+            for (hd_ctx* c : e->m) c->sed_settled = true;
+        }
+        if (k.bed) {
+            bed_settled = true;
+            for (hd_ctx* c : e->m) c->bed_settled = true;
+        }
         return k;
     };
     int first = (!settled && nsteps > 0) ? 1 : 0;

@@ -119,15 +119,15 @@ void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStre
 void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
-void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s);
+void hd_launch_vth_hesok(const HdBatch& b, int ntiles, hipStream_t s);
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s);
 void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);
 void hd_launch_fs_run(const HdBatch& b, int which, double s_start, int ketdinh, int nblocks, int spec,
                       hipStream_t s);
-void hd_launch_update_fs(const HdBatch& b, hipStream_t s);
-void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s);
-void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s);
+void hd_launch_update_fs(const HdBatch& b, int ntiles, hipStream_t s);
+void hd_launch_calc_qb(const HdBatch& b, int qb_mode, int ntiles, hipStream_t s);
+void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, int ntiles, hipStream_t s);
 void hd_launch_wet_count(const HdBatch& b, hipStream_t s);
 void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s);
 void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, hipStream_t s);

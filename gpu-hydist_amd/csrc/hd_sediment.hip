@@ -14,6 +14,21 @@
 
 namespace {
 
+/* Cell (i, j) of a thread over the context's active 64-column row tiles
+ * (tiles holding or bordering a cell with khouot != 2): outside them the
+ * sediment kernels that run by tiles have reached a fixed point once they
+ * have run over the whole grid, which the caller arranges. */
+__device__ __forceinline__ bool tile_cell(const HdDevG& d, int& i, int& j)
+{
+    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= d.ntiles[0]) return false;
+    const int tile = d.tiles[t];
+    const int ncb = (d.M + 3 + 63) >> 6;
+    i = tile / ncb;
+    j = (tile % ncb) * 64 + (threadIdx.x & 63);
+    return j < d.M + 3 && i < d.N + 3;
+}
+
 /* source_chs, Sediment_Transport.cu:12-49. */
 __device__ double source_chs(const hd_params& P, double wsm, double Zf, double fs, double fw, double vth, double h_moi)
 {
@@ -40,11 +55,16 @@ __device__ double source_nchs(const hd_params& P, double wsm, double Zf, double 
 }
 
 /* Find_VTH (Sediment_Transport.cu:541-568) and hesoK (:508-539). */
-__global__ void __launch_bounds__(256) k_vth_hesok(const HdDev* __restrict__ dv)
+__global__ void __launch_bounds__(256) k_vth_hesok(const HdDev* __restrict__ dv, int tiled)
 {
     const HdDevG& d = dev_view(dv);
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int i = blockIdx.y;
+    int j, i;
+    if (tiled) {
+        if (!tile_cell(d, i, j)) return;
+    } else {
+        j = blockIdx.x * blockDim.x + threadIdx.x;
+        i = blockIdx.y;
+    }
     int N = d.N, M = d.M, W = d.W;
     const hd_params& P = d.P;
     if (j > M || i > N) return;
@@ -603,22 +623,34 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
 }
 
 /* Update_FS, Sediment_Transport.cu:620-629. */
-__global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv)
+__global__ void __launch_bounds__(256) k_update_fs(const HdDev* __restrict__ dv, int tiled)
 {
     const HdDevG& d = dev_view(dv);
-    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    int i = blockIdx.y + 1;
+    int j, i;
+    if (tiled) {
+        if (!tile_cell(d, i, j)) return;
+        if (i < 1 || j < 1) return;
+    } else {
+        j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+        i = blockIdx.y + 1;
+    }
     if (i > d.N || j > d.M) return;
     d.FS[i * d.W + j] = d.tFS[i * d.W + j];
 }
 
 /* Calculate_Qb, Sediment_Transport.cu:341-414 (qb_mode REFERENCE keeps the
  * `!VTH` test as written, FIXED uses VTH != 0; corner copies braced). */
-__global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, int qb_mode)
+__global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, int qb_mode, int tiled)
 {
     const HdDevG& d = dev_view(dv);
-    int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    int i = blockIdx.y + 2;
+    int j, i;
+    if (tiled) {
+        if (!tile_cell(d, i, j)) return;
+        if (i < 2 || j < 2) return;
+    } else {
+        j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+        i = blockIdx.y + 2;
+    }
     int N = d.N, M = d.M, W = d.W;
     if (i > N || j > M) return;
     const hd_params& P = d.P;
@@ -658,25 +690,36 @@ __global__ void __launch_bounds__(256) k_calc_qb(const HdDev* __restrict__ dv, i
 /* _bed_load via BedLoad, Sediment_Transport.cu:416-506 / 630-639: rows
  * [sidx, eidx-1], cells of row runs with last - first >= 2 (t_u[pos > 0]
  * reads t_u[1], as written). */
-__global__ void __launch_bounds__(256) k_bedload(const HdDev* __restrict__ dv, int ketdinh, int slo, int eoff)
+__global__ void __launch_bounds__(256) k_bedload(const HdDev* __restrict__ dv, int ketdinh, int slo, int eoff, int tiled)
 {
     const HdDevG& d = dev_view(dv);
     const int sidx = slo, eidx = d.N - eoff;
-    __shared__ int s_moc;
-    __shared__ int s_dau[24], s_cuoi[24];
-    int i = sidx + blockIdx.y;
     int N = d.N, M = d.M, W = d.W, SL = d.SL;
-    if (i >= eidx) return;
-    if (threadIdx.x == 0) s_moc = min(d.moci[i], SL);
-    __syncthreads();
-    if (threadIdx.x < s_moc) { s_dau[threadIdx.x] = d.daui[i * SL + threadIdx.x]; s_cuoi[threadIdx.x] = d.cuoii[i * SL + threadIdx.x]; }
-    __syncthreads();
-    int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    if (j > M) return;
+    int i, j;
     int first = 0, last = 0;
     bool found = false;
-    for (int s = 0; s < s_moc; ++s)
-        if (s_dau[s] <= j && j <= s_cuoi[s]) { first = s_dau[s]; last = s_cuoi[s]; found = true; break; }
+    if (tiled) { /* run cells lie in the active tiles; the row's table is wave-uniform */
+        if (!tile_cell(d, i, j)) return;
+        if (i < sidx || i >= eidx || j < 2 || j > M) return;
+        const int moc = min(d.moci[i], SL);
+        for (int s = 0; s < moc; ++s) {
+            const int a = d.daui[i * SL + s], b = d.cuoii[i * SL + s];
+            if (a <= j && j <= b) { first = a; last = b; found = true; break; }
+        }
+    } else {
+        __shared__ int s_moc;
+        __shared__ int s_dau[24], s_cuoi[24];
+        i = sidx + blockIdx.y;
+        if (i >= eidx) return;
+        if (threadIdx.x == 0) s_moc = min(d.moci[i], SL);
+        __syncthreads();
+        if (threadIdx.x < s_moc) { s_dau[threadIdx.x] = d.daui[i * SL + threadIdx.x]; s_cuoi[threadIdx.x] = d.cuoii[i * SL + threadIdx.x]; }
+        __syncthreads();
+        j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+        if (j > M) return;
+        for (int s = 0; s < s_moc; ++s)
+            if (s_dau[s] <= j && j <= s_cuoi[s]) { first = s_dau[s]; last = s_cuoi[s]; found = true; break; }
+    }
     if (!found || last - first < 2) return;
     const hd_params& P = d.P;
     const double *FS = d.FS, *Hm = d.H_moi, *t_u = d.t_u, *t_v = d.t_v, *VTH = d.VTH, *Htdu = d.Htdu, *Htdv = d.Htdv,
@@ -723,10 +766,16 @@ __global__ void __launch_bounds__(256) k_bedload(const HdDev* __restrict__ dv, i
 
 } // namespace
 
-void hd_launch_vth_hesok(const HdBatch& b, hipStream_t s)
+static inline dim3 tile_grid(const HdBatch& b, int ntiles) { return dim3((ntiles + 3) / 4, 1, b.n); }
+
+void hd_launch_vth_hesok(const HdBatch& b, int ntiles, hipStream_t s)
 {
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(k_vth_hesok, tile_grid(b, ntiles), dim3(256), 0, s, b.dv, 1);
+        return;
+    }
     dim3 g2((b.M + 1 + 255) / 256, b.N + 1, b.n);
-    hipLaunchKernelGGL(k_vth_hesok, g2, dim3(256), 0, s, b.dv);
+    hipLaunchKernelGGL(k_vth_hesok, g2, dim3(256), 0, s, b.dv, 0);
 }
 
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s)
@@ -745,21 +794,33 @@ void hd_launch_fs_solve(const HdBatch& b, int which, hipStream_t s)
     else hipLaunchKernelGGL(k_fs_solve<false>, dim3((nl + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv);
 }
 
-void hd_launch_update_fs(const HdBatch& b, hipStream_t s)
+void hd_launch_update_fs(const HdBatch& b, int ntiles, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_fs, dim3((b.M + 255) / 256, b.N, b.n), dim3(256), 0, s, b.dv);
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(k_update_fs, tile_grid(b, ntiles), dim3(256), 0, s, b.dv, 1);
+        return;
+    }
+    hipLaunchKernelGGL(k_update_fs, dim3((b.M + 255) / 256, b.N, b.n), dim3(256), 0, s, b.dv, 0);
 }
 
-void hd_launch_calc_qb(const HdBatch& b, int qb_mode, hipStream_t s)
+void hd_launch_calc_qb(const HdBatch& b, int qb_mode, int ntiles, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_calc_qb, dim3((b.M - 1 + 255) / 256, b.N - 1, b.n), dim3(256), 0, s, b.dv, qb_mode);
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(k_calc_qb, tile_grid(b, ntiles), dim3(256), 0, s, b.dv, qb_mode, 1);
+        return;
+    }
+    hipLaunchKernelGGL(k_calc_qb, dim3((b.M - 1 + 255) / 256, b.N - 1, b.n), dim3(256), 0, s, b.dv, qb_mode, 0);
 }
 
-void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, hipStream_t s)
+void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, int ntiles, hipStream_t s)
 {
+    if (ntiles > 0) {
+        hipLaunchKernelGGL(k_bedload, tile_grid(b, ntiles), dim3(256), 0, s, b.dv, ketdinh, slo, eoff, 1);
+        return;
+    }
     int nr = b.N - eoff - slo;
     if (nr > 0)
-        hipLaunchKernelGGL(k_bedload, dim3((b.M - 1 + 255) / 256, nr, b.n), dim3(256), 0, s, b.dv, ketdinh, slo, eoff);
+        hipLaunchKernelGGL(k_bedload, dim3((b.M - 1 + 255) / 256, nr, b.n), dim3(256), 0, s, b.dv, ketdinh, slo, eoff, 0);
 }
 
 void hd_launch_fs_line(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s)

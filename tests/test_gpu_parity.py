@@ -80,6 +80,22 @@ def test_halfsteps_and_steps_bitwise(make, nsteps):
             assert_same_state(o, g, f"step {k + 2}")
 
 
+def test_sediment_starting_mid_run_bitwise():
+    """Suspended sediment switched on after 12 steps and the bed change
+    after 24, in one hd_run: the first sweep and the first bed-load flux run
+    over the whole grid, later ones over the active tiles; every field,
+    including VTH, Kx/Ky, Qbx/Qby, FS/tFS on land, matches the checker."""
+    case = synth.config(3, scale=0.25)
+    case.sediment_start = 12 * case.dT
+    case.bed_change_start = 24 * case.dT
+    case.bed_mode = BED_EVERY_STEP
+    o, g = pair(case)
+    op = case.opts()
+    o.run(0.0, 40, op)
+    g.run(0.0, 40, op)
+    assert_same_state(o, g, "sediment from step 13, bed change from step 25")
+
+
 def test_bed_change_reference_mode():
     case = small_river_sediment(qb_mode=QB_REFERENCE)
     o, g = pair(case)
@@ -147,8 +163,8 @@ def test_analytic_channel_boundaries():
 
 
 def test_long_runs_bitwise():
-    """Runs of up to 297 cells (the 300 x 60 channel's UZ lines): the packed
-    run solver's multi-chunk path (254 rows per chunk) against the checker."""
+    """Runs of up to 297 cells (the 300 x 60 channel's UZ lines): the run
+    solver's multi-chunk path (510 rows per chunk) against the checker."""
     case = synth.channel_case(N=300, M=60, nsteps=40, name="channel_300x60")
     o, g = pair(case)
     op = case.opts()
