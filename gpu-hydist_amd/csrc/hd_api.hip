@@ -151,7 +151,8 @@ struct Exec {
     }
 };
 
-/* solver grid: rb_mult blocks per line, at most one per table entry */
+/* solver grid: rb_mult blocks per line, at most one per table entry, in
+ * whole rounds of the eight XCDs (the solvers deal table slices by XCD) */
 static void set_knobs(hd_ctx* c, int rb_mult, int bw)
 {
     int N = c->N, M = c->M, SL = c->SL;
@@ -159,6 +160,8 @@ static void set_knobs(hd_ctx* c, int rb_mult, int bw)
     c->bw = bw;
     c->run_blocks_u = rb_mult * (M + 3) < (M + 3) * SL ? rb_mult * (M + 3) : (M + 3) * SL;
     c->run_blocks_v = rb_mult * (N + 3) < (N + 3) * SL ? rb_mult * (N + 3) : (N + 3) * SL;
+    c->run_blocks_u = (c->run_blocks_u + 7) & ~7;
+    c->run_blocks_v = (c->run_blocks_v + 7) & ~7;
 }
 
 static Exec exec_of(hd_ctx* c)

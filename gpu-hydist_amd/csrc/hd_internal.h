@@ -87,6 +87,27 @@ __device__ __forceinline__ const HdDevG& dev_view(const HdDev* dv)
     return reinterpret_cast<const HdDevG*>(dv)[blockIdx.z];
 }
 
+/* The entries of a packed run table (cnt of them) that this workgroup of a
+ * one-wave-per-run solver takes: start, end and stride of its loop.
+ * Workgroups are dealt to the eight XCDs in turn (workgroup b to XCD b % 8)
+ * and neighbouring table entries are neighbouring lines (each block of the
+ * table scan reserves its lines' entries together), so XCD x takes the x-th
+ * eighth of the table: lines that share cache lines are solved behind the
+ * same L2 instead of fetching each line from HBM eight times. */
+struct RunSlice {
+    int start, end, step;
+};
+__device__ __forceinline__ RunSlice run_slice(int cnt)
+{
+    const int G = (int)gridDim.x;
+    const int nx = G < 8 ? 1 : 8;
+    const int x = (int)blockIdx.x % nx, slot = (int)blockIdx.x / nx;
+    const int per = (G - x + nx - 1) / nx; /* workgroups on XCD x */
+    const int chunk = (cnt + nx - 1) / nx;
+    const int lo = x * chunk;
+    return RunSlice{lo + slot, cnt < lo + chunk ? cnt : lo + chunk, per};
+}
+
 
 /* A launch over a batch of contexts (one or an ensemble): dv points to n
  * device copies of HdDev, kernels take theirs from dv[blockIdx.z]; N, M are

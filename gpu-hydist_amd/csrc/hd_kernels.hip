@@ -922,7 +922,8 @@ __global__ void __launch_bounds__(64) k_adi_run(const HdDev* __restrict__ dv, in
     const double f4 = isU ? 2 * d.P.g * d.P.dTchia2dX : 2 * d.P.g * d.P.dTchia2dY;
     double* tz = d.t_z;
     double* tq = isU ? d.t_u : d.t_v;
-    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+    const RunSlice sl = run_slice(cnt);
+    for (int ri = sl.start; ri < sl.end; ri += sl.step) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
         if (line < sidx || line > eidx) continue;
@@ -1124,7 +1125,8 @@ __global__ void __launch_bounds__(64, 3) k_adi_spec(const HdDev* __restrict__ dv
     double* tq = isU ? d.t_u : d.t_v;
     const double* hh = d.h;
     const int nbr = (lane + 1) & (kWave - 1);
-    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+    const RunSlice sl = run_slice(cnt);
+    for (int ri = sl.start; ri < sl.end; ri += sl.step) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
         if (line < sidx || line > eidx) continue;
@@ -2239,18 +2241,30 @@ __device__ __forceinline__ int col_group_scan(const HdDevG& d, int g, const unsi
         const int f = dry ? dry[p] : 0;
         return k == 0 && !f;
     };
-    /* rows 2 .. L in passes of 1024 (the loads of a pass do not wait for any
-     * other value); the column's wetness at L-2 and L-1 for its end rule */
+    /* the rows of the union of the group's column extents (outside them
+     * khouot == 2 for good), in passes of 1024 (the loads of a pass do not
+     * wait for any other value); the column's wetness at L-2 and L-1 for
+     * its end rule */
     const int wl2 = wet(L - 2), wl1 = wet(L - 1);
+    int elo = 1 << 30, ehi = -1;
+    if (colok) {
+        const int2 e = d.ext[N + 3 + j];
+        if (e.x <= e.y) { elo = e.x; ehi = e.y; }
+    }
+    for (int o = 1; o < kColG; o <<= 1) { /* the group's 8 columns are 8 adjacent lanes */
+        elo = min(elo, __shfl_xor(elo, o, kWave));
+        ehi = max(ehi, __shfl_xor(ehi, o, kWave));
+    }
     if (c == 0) { s_ts[q] = 0; s_te[q] = 0; s_pb[q] = 0; }
-    const int xhi = L;
-    for (int x0 = 2; x0 <= xhi; x0 += 32 * kColCh) {
+    __syncthreads(); /* an empty extent runs no pass: the counts are read as set here */
+    const int xhi = ehi < L ? ehi : L;
+    for (int x0 = elo > 2 ? elo : 2; x0 <= xhi; x0 += 32 * kColCh) {
         const int xs = x0 + 32 * c;
         int kv[32];
         unsigned char dv_[32];
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const int x = xs + r <= L ? xs + r : L;
+            const int x = xs + r <= xhi ? xs + r : xhi;
             kv[r] = kh[x * W + jc];
             dv_[r] = dry ? dry[x * W + jc] : 0;
         }
@@ -2310,7 +2324,8 @@ __device__ __forceinline__ int col_group_scan(const HdDevG& d, int g, const unsi
  * with one atomic.  Blocks [0, nrowb): rows 4b .. 4b+3, one wave per row;
  * with count_wet these waves also add the row's wet cells (rows 0..N+2) to
  * the wet-cell accumulator.  Blocks [nrowb, nrowb + ncolb): columns
- * 2 + 8b' .. 2 + 8b' + 7 (col_group_scan).  Block nrowb + ncolb: the FS
+ * 2 + 8g .. 2 + 8g + 7 (col_group_scan) for a group g that the block index
+ * permutes.  Block nrowb + ncolb: the FS
  * edge interpolations.  `dry` (the drying flags, when the state copy runs in
  * the same launch) marks cells that count as dry already. */
 __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int count_wet,
@@ -2326,7 +2341,11 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
     const bool rows = bid < nrowb;
     if (!rows) { /* kColG columns per block */
         __shared__ int s_cf[kColG * 24], s_cl[kColG * 24], s_cn[kColG], s_cbase[kColG];
-        const int g = bid - nrowb;
+        /* four adjacent groups per XCD (block b runs on XCD b % 8; nrowb and
+         * ncolb are multiples of 8 and 32): the 128-byte mask lines that
+         * they share come into one L2 */
+        const int k = bid - nrowb, x = k & 7, sl = k >> 3;
+        const int g = ((sl >> 2) * 8 + x) * 4 + (sl & 3);
         const int n = col_group_scan(d, g, dry, s_cf, s_cl);
         const int q = threadIdx.x % kColG, c = threadIdx.x / kColG;
         if (c == 0) s_cn[q] = n;
@@ -2500,9 +2519,18 @@ bool hd_uvz_segments_fits(const HdBatch& b)
     return b.M - 1 <= 8 * 256 && imax <= 8 * 256;
 }
 
+/* table-scan blocks: rows 4 per block, columns kColG per block; counts
+ * padded to whole rounds of the eight XCDs (idle blocks find no line) */
+static void scan_blocks(const HdBatch& b, int& nrowb, int& ncolb)
+{
+    nrowb = (((b.N + 3 + 3) / 4) + 7) & ~7;
+    ncolb = (((b.M - 1 + kColG - 1) / kColG) + 31) & ~31;
+}
+
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s)
 {
-    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + kColG - 1) / kColG; /* rows: 4 per block; columns: kColG */
+    int nrowb, ncolb;
+    scan_blocks(b, nrowb, ncolb);
     int ncb = (b.M + 3 + 63) / 64;
     int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ncb;
     hipLaunchKernelGGL(k_uvz_segments, dim3(nrowb + ncolb + 1 + (tiles + 3) / 4, 1, b.n), dim3(256), 0, s, b.dv, kd,
@@ -2511,7 +2539,8 @@ void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int co
 
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s)
 {
-    int nrowb = (b.N + 3 + 3) / 4, ncolb = (b.M - 1 + kColG - 1) / kColG; /* rows: 4 per block; columns: kColG */
+    int nrowb, ncolb;
+    scan_blocks(b, nrowb, ncolb);
     /* + one block for the FS edge interpolations */
     hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb + 1, 1, b.n), dim3(256), 0, s, b.dv, nrowb, ncolb, count);
 }

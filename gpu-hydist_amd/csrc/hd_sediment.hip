@@ -425,7 +425,8 @@ __global__ void __launch_bounds__(64) k_fs_run(const HdDev* __restrict__ dv, dou
     const int cnt = d.run_cnt[isJ ? 0 : 1] < cap ? d.run_cnt[isJ ? 0 : 1] : cap;
     const int4* runs = isJ ? d.runs_u : d.runs_v;
     const double t = d.tdev[0];
-    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+    const RunSlice sl = run_slice(cnt);
+    for (int ri = sl.start; ri < sl.end; ri += sl.step) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
         if (line < 3 || line > lmax || last < first + 2) continue;
@@ -539,7 +540,8 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
     const int cnt = d.run_cnt[isJ ? 0 : 1] < cap ? d.run_cnt[isJ ? 0 : 1] : cap;
     const int4* runs = isJ ? d.runs_u : d.runs_v;
     const double t = d.tdev[0];
-    for (int ri = blockIdx.x; ri < cnt; ri += gridDim.x) {
+    const RunSlice sl = run_slice(cnt);
+    for (int ri = sl.start; ri < sl.end; ri += sl.step) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
         if (line < 3 || line > lmax || last < first + 2) continue;
