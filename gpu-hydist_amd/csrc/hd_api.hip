@@ -353,13 +353,7 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
         size_t ncb = (size_t)(M + 3 + 63) / 64;
         if ((rc = dalloc(c, (void**)&d.tiles, (size_t)(N + 3) * ncb * 4))) { hd_destroy(c); return rc; }
         if ((rc = dalloc(c, (void**)&d.ntiles, 4))) { hd_destroy(c); return rc; }
-        /* line extents: the whole line until the mask is known */
-        std::vector<int2> ext((size_t)(N + 3) + (M + 3), make_int2(0, 1 << 30));
-        if ((rc = dalloc(c, (void**)&d.ext, ext.size() * sizeof(int2)))) { hd_destroy(c); return rc; }
-        if (hipMemcpy(d.ext, ext.data(), ext.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
-            hd_destroy(c);
-            return fail(nullptr, HD_EHIP, "hipMemcpy failed");
-        }
+        if ((rc = dalloc(c, (void**)&d.wetb, (size_t)(N + 3) * ncb * 8))) { hd_destroy(c); return rc; }
     }
     c->t_u0 = d.t_u;
     c->t_v0 = d.t_v;
@@ -514,18 +508,6 @@ static int compute_tiles(hd_ctx* c)
     std::vector<int> list;
     for (size_t t = 0; t < live.size(); ++t)
         if (live[t]) list.push_back((int)t);
-    /* per-line extents of the cells that can ever be wet (khouot != 2 stays
-     * so: only khouot == 1 cells reopen) */
-    std::vector<int2> ext((size_t)(N + 3) + (M + 3), make_int2(1, 0));
-    for (int i = 0; i < N + 3; ++i)
-        for (int j = 0; j < M + 3; ++j)
-            if (kh[(size_t)i * W + j] != 2) {
-                int2& r = ext[i];
-                if (r.x > r.y) r = make_int2(j, j); else r.y = j;
-                int2& q = ext[(size_t)(N + 3) + j];
-                if (q.x > q.y) q = make_int2(i, i); else { q.x = std::min(q.x, i); q.y = std::max(q.y, i); }
-            }
-    HIPCHK(c, hipMemcpyAsync(c->d.ext, ext.data(), ext.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     int n = (int)list.size();
     if (n) HIPCHK(c, hipMemcpyAsync(c->d.tiles, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d.ntiles, &n, 4, hipMemcpyHostToDevice, c->stream));

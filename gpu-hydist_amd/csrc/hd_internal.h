@@ -35,11 +35,11 @@
  * runs_u / runs_v packed wet runs (line, first, last, 0) of the UZ / VZ
  * sweeps and run_cnt their entry counts; flags [0] segment overflow, [1]
  * boundary range; wet_count [0] run accumulator, [1] one-shot count; tdev
- * the device time; tiles / ntiles the active 64-column row tiles; ext the
- * first and last position of each row (0 .. N+2) and then each column
- * (0 .. M+2) holding a cell with khouot != 2 (outside it no cell is ever
- * wet); cand_cnt / cand the wetting candidates (64 per line); dryflag one
- * byte per cell. */
+ * the device time; tiles / ntiles the active 64-column row tiles; wetb the
+ * wet bits of the segment finder, one 64-bit word per 64-cell tile row
+ * ((N+3) x ceil((M+3)/64) words, bit b of word (i, w) for cell
+ * (i, 64w + b): khouot == 0 and no drying flag); cand_cnt / cand the
+ * wetting candidates (64 per line); dryflag one byte per cell. */
 #define HD_DEV_MEMBERS(G) \
     int N, M, W, SL, total_time, series_len; \
     hd_params P; \
@@ -61,7 +61,7 @@
     G double* tdev; \
     G int* tiles; \
     G int* ntiles; \
-    G int2* ext; \
+    G unsigned long long* wetb; \
     G int *cand_cnt, *cand; \
     G unsigned char* dryflag;
 
@@ -119,7 +119,7 @@ struct HdBatch {
 
 /* kernel launchers (hd_kernels.hip, hd_sediment.hip), one kernel each */
 void hd_launch_onetime_init(const HdBatch& b, hipStream_t s);
-void hd_launch_find_limits(const HdBatch& b, hipStream_t s); /* run-table reset + segments (+ FS edges) */
+void hd_launch_find_limits(const HdBatch& b, hipStream_t s); /* wet bits from khouot, run-table reset, segments (+ FS edges) */
 bool hd_uvz_segments_fits(const HdBatch& b);
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s);
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s); /* also the FS edge interpolations */

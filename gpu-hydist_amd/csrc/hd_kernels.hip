@@ -25,12 +25,6 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kCand = 64;   /* wetting candidates kept per line */
 
-__device__ __forceinline__ unsigned long long lanemask_lt()
-{
-    unsigned lane = __lane_id();
-    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
 /* ================================================================ init */
 
 /* Onetime_init, support_funcs.cu:20-58: every cell ends as khouot = 2 except
@@ -1375,6 +1369,10 @@ __global__ void __launch_bounds__(256) k_normalize(const HdDev* __restrict__ dv,
         dry = inrange && (hm <= d.P.H_TINH);
     }
     dryflag[p] = dry;
+    {   /* the wet bits of this 64-cell tile row (the wave) */
+        const unsigned long long bits = __ballot(kh == 0 && !dry);
+        if ((threadIdx.x & 63) == 0) d.wetb[(size_t)i * ((M + 3 + 63) >> 6) + (j >> 6)] = bits;
+    }
     /* a position can open a neighbour only where t_z > z; t_z changes
      * during the wetting pass only at cells it opens, and those are reached
      * by the pass's own cascade */
@@ -1865,7 +1863,12 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
         kh[pq] = 0;
         FS[pq] = FS[ps];
         bool inrange = horizontal ? (q >= 2 && q <= M && line >= 1 && line <= N) : (q >= 2 && q <= N && line >= 1 && line <= M);
-        dryflag[pq] = inrange && (Hm[pq] <= d.P.H_TINH);
+        const bool dr = inrange && (Hm[pq] <= d.P.H_TINH);
+        dryflag[pq] = dr;
+        if (!dr) { /* the opened cell counts as wet for the segment finder */
+            const int r = horizontal ? line : q, cc = horizontal ? q : line;
+            atomicOr((unsigned long long*)d.wetb + (size_t)r * ((M + 3 + 63) >> 6) + (cc >> 6), 1ull << (cc & 63));
+        }
     };
     /* visit positions in scan order: the recorded ones, and after a cell
      * opens its right neighbour, that neighbour (the reference scan's cascade) */
@@ -2053,65 +2056,84 @@ __global__ void k_reset_runs(const HdDev* __restrict__ dv)
     if (threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
 }
 
-/* Wet runs of one line (a row when ROW, else a column), positions 2..L, by
- * one wave: the line is read eight 64-cell chunks at a time (all loads of a
- * batch in flight together), neighbours come from lane shuffles, starts and
- * ends are compacted with ballots.  The rule is the serial scan's
- * (support_funcs.cu:252-279): a run must start at <= L-2; it ends at its last
- * wet cell, except that a run reaching L-1 or L only counts when L-2 is wet.
- * Entries k < SL go to dau / cuoi (global) and sf / sl (LDS); returns the
- * number of runs found. */
-template <bool ROW>
-__device__ __forceinline__ int scan_runs(const int* kh, int W, int line, int L, int SL, int lane, int* dau,
-                                         int* cuoi, int* sf, int* sl, int2 ext, const unsigned char* dry)
+/* Wet bits: wetb[i * ncb + w] (ncb = (M + 3 + 63) / 64), bit b, is set
+ * when cell (i, 64 w + b) has khouot == 0 and no drying flag, i.e. the cells
+ * the segment finder counts as wet.  The drying pass (k_normalize with post)
+ * writes every word of the cells it covers, one ballot per 64-cell tile row,
+ * and the wetting pass sets the bits of the cells it opens; k_wet_bits
+ * builds them from khouot alone after init and restarts.  Words of tiles
+ * that hold only khouot == 2 cells stay zero. */
+
+/* bits 0 .. n of a word (none for n < 0, all for n >= 63) */
+__device__ __forceinline__ unsigned long long bits_upto(int n)
 {
-    /* wet: khouot == 0 and, when the state copy of this half-step runs in
-     * the same launch, not about to be dried by it */
-    auto wetv = [&](int x) -> int {
-        if (x < 2 || x > L) return 0;
-        const int q = ROW ? line * W + x : x * W + line;
-        const int k = kh[q];
-        const int f = dry ? dry[q] : 0; /* loaded with khouot, not after it */
-        return k == 0 && !f;
+    return n < 0 ? 0ull : (n >= 63 ? ~0ull : ((2ull << n) - 1));
+}
+
+/* Starts and ends of the wet runs among positions base .. base + 63 of a
+ * line (the rule of the serial scan, support_funcs.cu:252-279: a run starts
+ * at a wet position <= L-2 whose predecessor is dry; it ends at its last wet
+ * position, where an end at L-1 counts only when L-2 is wet and an end at L
+ * only when L-2 and L-1 are).  m: the positions' wet bits, already limited
+ * to 2 .. L; prev / next: wetness of base - 1 and base + 64. */
+struct RunBits {
+    unsigned long long st, en;
+};
+__device__ __forceinline__ RunBits run_bits(unsigned long long m, int base, int L, unsigned prev, unsigned next,
+                                            unsigned wl2, unsigned wl1)
+{
+    const unsigned long long le2 = bits_upto(L - 2 - base);
+    unsigned long long extra = 0;
+    if (wl2 && L - 1 >= base && L - 1 < base + 64) extra |= 1ull << (L - 1 - base);
+    if (wl2 && wl1 && L >= base && L < base + 64) extra |= 1ull << (L - base);
+    RunBits r;
+    r.st = m & ~((m << 1) | (unsigned long long)prev) & le2;
+    r.en = m & ~((m >> 1) | ((unsigned long long)next << 63)) & (le2 | extra);
+    return r;
+}
+
+/* The runs of row i (positions 2..M) by one wave from its wet bits: a word
+ * per lane (64 words = 4096 cells per batch), neighbours' edge bits by
+ * shuffles, ranks by a wave prefix sum.  Entries k < SL go to dau / cuoi
+ * (global) and sf / sl (LDS); returns the number of runs found. */
+__device__ __forceinline__ int row_word_scan(const HdDevG& d, int i, int lane, int* dau, int* cuoi, int* sf, int* sl)
+{
+    const int M = d.M, L = M, SL = d.SL;
+    const int ncb = (M + 3 + 63) >> 6;
+    const auto row = d.wetb + (size_t)i * ncb;
+    auto wetx = [&](int x) -> unsigned {
+        return (x >= 2 && x <= L) ? (unsigned)((row[x >> 6] >> (x & 63)) & 1) : 0u;
     };
-    const int wLm2 = wetv(L - 2), wLm1 = wetv(L - 1);
+    const unsigned wl2 = wetx(L - 2), wl1 = wetx(L - 1);
+    unsigned carry = 0; /* wetness of the position before the batch */
     int ns = 0, ne = 0;
-    int carry = 0; /* wet(first position of the chunk - 1) */
-    constexpr int kB = 16; /* chunks of 64 cells per batch, all loads in flight together */
-    /* only positions inside the line's extent can be wet; before it the
-     * previous position is dry, as carry says */
-    const int x0 = ext.x > 2 ? ext.x : 2, x1 = ext.y < L ? ext.y : L;
-    for (int b0 = x0; b0 <= x1; b0 += kB * kWave) {
-        int w[kB];
-#pragma unroll
-        for (int u = 0; u < kB; ++u) w[u] = wetv(b0 + kWave * u + lane);
-        const int after = wetv(b0 + kB * kWave); /* the cell after the batch */
-#pragma unroll
-        for (int u = 0; u < kB; ++u) {
-            if (b0 + kWave * u > x1) break;
-            const int x = b0 + kWave * u + lane;
-            const int up = __shfl_up(w[u], 1, kWave);
-            const int dn = __shfl_down(w[u], 1, kWave);
-            const int nx0 = (u < kB - 1) ? __shfl(w[u < kB - 1 ? u + 1 : u], 0, kWave) : after;
-            const int last_of_chunk = __shfl(w[u], kWave - 1, kWave);
-            const int prevw = lane == 0 ? carry : up;
-            const int nextw = lane == kWave - 1 ? nx0 : dn;
-            carry = last_of_chunk;
-            const bool w0 = w[u] != 0;
-            const bool st = w0 && (x == 2 || !prevw) && x <= L - 2;
-            const bool en = w0 && !nextw && (x <= L - 2 || (x == L - 1 && wLm2) || (x == L && wLm1 && wLm2));
-            const unsigned long long sm = __ballot(st), em = __ballot(en);
-            if (st) {
-                int k = ns + __popcll(sm & lanemask_lt());
-                if (k < SL) { dau[k] = x; sf[k] = x; }
-            }
-            if (en) {
-                int k = ne + __popcll(em & lanemask_lt());
-                if (k < SL) { cuoi[k] = x; sl[k] = x; }
-            }
-            ns += __popcll(sm);
-            ne += __popcll(em);
+    for (int w0 = 0; w0 < ncb; w0 += kWave) {
+        const int w = w0 + lane, base = 64 * w;
+        unsigned long long m = w < ncb ? row[w] : 0ull;
+        m &= bits_upto(L - base) & ~bits_upto(1 - base);
+        const unsigned after = wetx(64 * (w0 + kWave));
+        const unsigned long long pw = __shfl_up(m, 1, kWave), nw = __shfl_down(m, 1, kWave);
+        const unsigned prev = lane == 0 ? carry : (unsigned)(pw >> 63);
+        const unsigned next = lane == kWave - 1 ? after : (unsigned)(nw & 1);
+        carry = (unsigned)(__shfl(m, kWave - 1, kWave) >> 63);
+        const RunBits rb = run_bits(m, base, L, prev, next, wl2, wl1);
+        const int cs = __popcll(rb.st), ce = __popcll(rb.en);
+        int ps = cs, pe = ce; /* inclusive prefix over the lanes */
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int a = __shfl_up(ps, o, kWave), b = __shfl_up(pe, o, kWave);
+            if (lane >= o) { ps += a; pe += b; }
         }
+        int ks = ns + ps - cs, ke = ne + pe - ce;
+        for (unsigned long long b = rb.st; b; b &= b - 1, ++ks) {
+            const int x = base + __ffsll(b) - 1;
+            if (ks < SL) { dau[ks] = x; sf[ks] = x; }
+        }
+        for (unsigned long long b = rb.en; b; b &= b - 1, ++ke) {
+            const int x = base + __ffsll(b) - 1;
+            if (ke < SL) { cuoi[ke] = x; sl[ke] = x; }
+        }
+        ns += __shfl(ps, kWave - 1, kWave);
+        ne += __shfl(pe, kWave - 1, kWave);
     }
     return ns;
 }
@@ -2211,103 +2233,108 @@ __device__ void fs_edges_block(const HdDevG& d, const unsigned char* dry, int kd
     }
 }
 
-/* The column tables of Find_Calculation_limits_Vertical (the run rule of
- * scan_runs, support_funcs.cu:252-279) for kColG adjacent columns per block
- * of 256 threads: thread (chunk c, column q) turns 32 rows of its column into
- * a wet bit mask -- a wave's loads are 8 adjacent columns of 8 rows, instead
- * of 64 rows of one column, all issued before any is used -- the
- * neighbouring chunks' edge bits come through LDS, starts and ends are bit
- * operations on the masks and their ranks prefix sums over the chunks.  Rows
- * beyond 1024 are done in further passes with the counts and the last bit
- * carried. */
-constexpr int kColG = 8;                 /* columns per block */
-constexpr int kColCh = 256 / kColG;      /* chunks of 32 rows per pass */
-
-__device__ __forceinline__ int col_group_scan(const HdDevG& d, int g, const unsigned char* dry, int* s_f, int* s_l)
+/* Transpose of a 64 x 64 bit block held one row per lane (bit q of lane r
+ * = element (r, q)): afterwards lane q holds column q (bit r = element
+ * (r, q)).  Six rounds; in round s the lanes r and r ^ s swap the
+ * off-diagonal s x s sub-blocks of their 2s x 2s block. */
+__device__ __forceinline__ unsigned long long transpose64(unsigned long long a, int lane)
 {
-    const int N = d.N, M = d.M, W = d.W, SL = d.SL, L = N;
-    const int q = threadIdx.x % kColG, c = threadIdx.x / kColG;
-    const int j = 2 + kColG * g + q;
-    const bool colok = j <= M;
-    const int jc = colok ? j : M; /* a column inside the grid for the loads of idle threads */
-    const int* kh = d.khouot;
-    __shared__ unsigned int s_m[kColCh][kColG];
-    __shared__ int s_cs[kColCh][kColG], s_ce[kColCh][kColG];
-    __shared__ int s_ts[kColG], s_te[kColG], s_pb[kColG];
-    auto wet = [&](int x) -> int {
-        if (!colok || x < 2 || x > L) return 0;
-        const int p = x * W + j;
-        const int k = kh[p];
-        const int f = dry ? dry[p] : 0;
-        return k == 0 && !f;
+    constexpr unsigned long long kKeep[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                             0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0, sh = 32; k < 6; ++k, sh >>= 1) {
+        const unsigned long long b = __shfl_xor(a, sh, kWave);
+        if (lane & sh) a ^= ((b >> sh) ^ a) & kKeep[k];
+        else a ^= (((a >> sh) ^ b) & kKeep[k]) << sh;
+    }
+    return a;
+}
+
+/* The column tables of Find_Calculation_limits_Vertical (the run rule of
+ * run_bits, support_funcs.cu:303-330) for the 64 columns of word column cb,
+ * by a block of four waves: in a pass of 1024 rows, lane r of wave c loads
+ * the words of rows 64 (4c + k) + r (k < 4; one word carries the 64
+ * columns), four transposes give each lane its column's bits of those rows,
+ * the waves' edge bits and run counts meet in LDS, ranks are prefix sums
+ * over the waves.  Rows beyond 1024 are done in further passes with the
+ * counts and the last bit carried.  Every thread returns its column's
+ * number of entries (min(runs, SL)); entries go to dauj / cuoij and to
+ * sf / sl [column * 24 + k]. */
+constexpr int kColKW = 4; /* words (64 rows each) per wave and pass */
+
+__device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, int* s_l)
+{
+    const int N = d.N, M = d.M, SL = d.SL, L = N;
+    const int ncb = (M + 3 + 63) >> 6;
+    const int lane = threadIdx.x & (kWave - 1), c = threadIdx.x / kWave;
+    const int j = 64 * cb + lane;
+    const bool colok = j >= 2 && j <= M;
+    const auto wb = d.wetb + cb; /* word (x, cb) at wb[x * ncb] */
+    auto wetx = [&](int x) -> unsigned {
+        return (colok && x >= 2 && x <= L) ? (unsigned)((wb[(size_t)x * ncb] >> lane) & 1) : 0u;
     };
-    /* the rows of the union of the group's column extents (outside them
-     * khouot == 2 for good), in passes of 1024 (the loads of a pass do not
-     * wait for any other value); the column's wetness at L-2 and L-1 for
-     * its end rule */
-    const int wl2 = wet(L - 2), wl1 = wet(L - 1);
-    int elo = 1 << 30, ehi = -1;
-    if (colok) {
-        const int2 e = d.ext[N + 3 + j];
-        if (e.x <= e.y) { elo = e.x; ehi = e.y; }
-    }
-    for (int o = 1; o < kColG; o <<= 1) { /* the group's 8 columns are 8 adjacent lanes */
-        elo = min(elo, __shfl_xor(elo, o, kWave));
-        ehi = max(ehi, __shfl_xor(ehi, o, kWave));
-    }
-    if (c == 0) { s_ts[q] = 0; s_te[q] = 0; s_pb[q] = 0; }
-    __syncthreads(); /* an empty extent runs no pass: the counts are read as set here */
-    const int xhi = ehi < L ? ehi : L;
-    for (int x0 = elo > 2 ? elo : 2; x0 <= xhi; x0 += 32 * kColCh) {
-        const int xs = x0 + 32 * c;
-        int kv[32];
-        unsigned char dv_[32];
+    __shared__ unsigned s_first[4][kWave], s_last[4][kWave], s_pb[kWave];
+    __shared__ int s_cs[4][kWave], s_ce[4][kWave], s_ts[kWave], s_te[kWave];
+    const unsigned wl2 = wetx(L - 2), wl1 = wetx(L - 1);
+    if (c == 0) { s_ts[lane] = 0; s_te[lane] = 0; s_pb[lane] = 0; }
+    __syncthreads();
+    constexpr int kPass = 4 * kColKW * kWave;
+    for (int x0 = 0; x0 <= L; x0 += kPass) {
+        unsigned long long t[kColKW], st[kColKW], en[kColKW];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const int x = xs + r <= xhi ? xs + r : xhi;
-            kv[r] = kh[x * W + jc];
-            dv_[r] = dry ? dry[x * W + jc] : 0;
+        for (int k = 0; k < kColKW; ++k) {
+            const int r = x0 + kWave * (kColKW * c + k) + lane;
+            t[k] = r <= L ? wb[(size_t)r * ncb] : 0ull;
         }
-        unsigned int m = 0;
+        const unsigned nxt = wetx(x0 + kPass);
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const int x = xs + r;
-            const bool w = colok && x <= xhi && kv[r] == 0 && !dv_[r];
-            m |= (unsigned int)w << r;
+        for (int k = 0; k < kColKW; ++k) {
+            const int base = x0 + kWave * (kColKW * c + k);
+            t[k] = transpose64(t[k], lane) & (colok ? (bits_upto(L - base) & ~bits_upto(1 - base)) : 0ull);
         }
-        s_m[c][q] = m;
+        s_first[c][lane] = (unsigned)(t[0] & 1);
+        s_last[c][lane] = (unsigned)(t[kColKW - 1] >> 63);
         __syncthreads();
-        const unsigned int prev = c > 0 ? (s_m[c - 1][q] >> 31) : (unsigned int)s_pb[q];
-        const unsigned int next = c < kColCh - 1 ? (s_m[c + 1][q] & 1u) : (unsigned int)wet(x0 + 32 * kColCh);
-        const int nle = L - 2 - xs + 1; /* rows of the chunk at or before L-2 */
-        const unsigned int le2 = nle >= 32 ? ~0u : (nle <= 0 ? 0u : ((1u << nle) - 1));
-        unsigned int extra = 0;
-        if (wl2 && L - 1 >= xs && L - 1 < xs + 32) extra |= 1u << (L - 1 - xs);
-        if (wl2 && wl1 && L >= xs && L < xs + 32) extra |= 1u << (L - xs);
-        const unsigned int st = m & ~((m << 1) | prev) & le2;
-        const unsigned int en = m & ~((m >> 1) | (next << 31)) & (le2 | extra);
-        s_cs[c][q] = __popc(st);
-        s_ce[c][q] = __popc(en);
-        __syncthreads();
-        int ks = s_ts[q], ke = s_te[q];
-        for (int k = 0; k < c; ++k) { ks += s_cs[k][q]; ke += s_ce[k][q]; }
-        for (unsigned int b = st; b; b &= b - 1, ++ks) {
-            const int x = xs + __ffs((int)b) - 1;
-            if (ks < SL) { d.dauj[j * SL + ks] = x; s_f[q * 24 + ks] = x; }
+        unsigned prev = c > 0 ? s_last[c - 1][lane] : s_pb[lane];
+        const unsigned after = c < 3 ? s_first[c + 1][lane] : nxt;
+        int cs = 0, ce = 0;
+#pragma unroll
+        for (int k = 0; k < kColKW; ++k) {
+            const int base = x0 + kWave * (kColKW * c + k);
+            const unsigned nk = k < kColKW - 1 ? (unsigned)(t[k + 1] & 1) : after;
+            const RunBits rb = run_bits(t[k], base, L, prev, nk, wl2, wl1);
+            prev = (unsigned)(t[k] >> 63);
+            st[k] = rb.st;
+            en[k] = rb.en;
+            cs += __popcll(rb.st);
+            ce += __popcll(rb.en);
         }
-        for (unsigned int b = en; b; b &= b - 1, ++ke) {
-            const int x = xs + __ffs((int)b) - 1;
-            if (ke < SL) { d.cuoij[j * SL + ke] = x; s_l[q * 24 + ke] = x; }
-        }
+        s_cs[c][lane] = cs;
+        s_ce[c][lane] = ce;
         __syncthreads();
-        if (c == kColCh - 1) {
-            s_ts[q] = ks;
-            s_te[q] = ke;
-            s_pb[q] = (int)(m >> 31);
+        int ks = s_ts[lane], ke = s_te[lane];
+        for (int w = 0; w < c; ++w) { ks += s_cs[w][lane]; ke += s_ce[w][lane]; }
+#pragma unroll
+        for (int k = 0; k < kColKW; ++k) {
+            const int base = x0 + kWave * (kColKW * c + k);
+            for (unsigned long long b = st[k]; b; b &= b - 1, ++ks) {
+                const int x = base + __ffsll(b) - 1;
+                if (ks < SL) { d.dauj[j * SL + ks] = x; s_f[lane * 24 + ks] = x; }
+            }
+            for (unsigned long long b = en[k]; b; b &= b - 1, ++ke) {
+                const int x = base + __ffsll(b) - 1;
+                if (ke < SL) { d.cuoij[j * SL + ke] = x; s_l[lane * 24 + ke] = x; }
+            }
+        }
+        __syncthreads(); /* the pass's counts are read before they are replaced */
+        if (c == 3) {
+            s_ts[lane] = ks;
+            s_te[lane] = ke;
+            s_pb[lane] = (unsigned)(t[kColKW - 1] >> 63);
         }
         __syncthreads();
     }
-    const int ns = s_ts[q];
+    const int ns = s_ts[lane];
     const int n = colok ? (ns < SL ? ns : SL) : 0;
     if (c == 0 && colok) {
         d.mocj[j] = n;
@@ -2316,18 +2343,18 @@ __device__ __forceinline__ int col_group_scan(const HdDevG& d, int g, const unsi
     return n;
 }
 
-/* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335):
- * the per-line tables daui / cuoii / moci (rows i = 2..N) and dauj / cuoij /
- * mocj (columns j = 2..M), plus the packed run tables the solvers read
- * (runs_v for rows, runs_u for columns; their counters were zeroed earlier
- * in the half-step).  Blocks of 256 threads; each reserves its table slots
- * with one atomic.  Blocks [0, nrowb): rows 4b .. 4b+3, one wave per row;
- * with count_wet these waves also add the row's wet cells (rows 0..N+2) to
- * the wet-cell accumulator.  Blocks [nrowb, nrowb + ncolb): columns
- * 2 + 8g .. 2 + 8g + 7 (col_group_scan) for a group g that the block index
- * permutes.  Block nrowb + ncolb: the FS
- * edge interpolations.  `dry` (the drying flags, when the state copy runs in
- * the same launch) marks cells that count as dry already. */
+/* Find_Calculation_limits_Horizontal / _Vertical (support_funcs.cu:230-335)
+ * from the wet bits: the per-line tables daui / cuoii / moci (rows
+ * i = 2..N) and dauj / cuoij / mocj (columns j = 2..M), plus the packed run
+ * tables the solvers read (runs_v for rows, runs_u for columns; their
+ * counters were zeroed earlier in the half-step).  Blocks of 256 threads;
+ * each reserves its table slots with one atomic.  Blocks [0, nrowb): rows
+ * 4b .. 4b+3, one wave per row; with count_wet these waves also add the
+ * row's wet cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
+ * [nrowb, nrowb + ncolb): the 64 columns of word column b - nrowb
+ * (col_word_scan).  Block nrowb + ncolb: the FS edge interpolations; `dry`
+ * (the drying flags, when the state copy runs in the same launch) marks
+ * the cells that copy zeroes. */
 __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int count_wet,
                                                   const unsigned char* dry, int kd, int kn)
 {
@@ -2335,61 +2362,48 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
         fs_edges_block(d, dry, kd, kn);
         return;
     }
-    const int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const int N = d.N, M = d.M, SL = d.SL;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int* kh = d.khouot;
-    const bool rows = bid < nrowb;
-    if (!rows) { /* kColG columns per block */
-        __shared__ int s_cf[kColG * 24], s_cl[kColG * 24], s_cn[kColG], s_cbase[kColG];
-        /* four adjacent groups per XCD (block b runs on XCD b % 8; nrowb and
-         * ncolb are multiples of 8 and 32): the 128-byte mask lines that
-         * they share come into one L2 */
-        const int k = bid - nrowb, x = k & 7, sl = k >> 3;
-        const int g = ((sl >> 2) * 8 + x) * 4 + (sl & 3);
-        const int n = col_group_scan(d, g, dry, s_cf, s_cl);
-        const int q = threadIdx.x % kColG, c = threadIdx.x / kColG;
-        if (c == 0) s_cn[q] = n;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int tot = 0;
-            for (int k = 0; k < kColG; ++k) tot += s_cn[k];
-            int base = tot ? atomicAdd(d.run_cnt + 0, tot) : 0;
-            for (int k = 0; k < kColG; ++k) { s_cbase[k] = base; base += s_cn[k]; }
+    const int ncb = (M + 3 + 63) >> 6;
+    if (bid >= nrowb) { /* 64 columns per block */
+        __shared__ int s_cf[kWave * 24], s_cl[kWave * 24], s_cbase[kWave];
+        const int cb = bid - nrowb;
+        const int n = col_word_scan(d, cb, s_cf, s_cl);
+        if (wv == 0) { /* slots of the block's columns in column order, one atomic */
+            int p = n;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int a = __shfl_up(p, o, kWave);
+                if (lane >= o) p += a;
+            }
+            const int tot = __shfl(p, kWave - 1, kWave);
+            int base = 0;
+            if (lane == 0 && tot) base = atomicAdd(d.run_cnt + 0, tot);
+            s_cbase[lane] = __shfl(base, 0, kWave) + p - n;
         }
         __syncthreads();
         const int cap = (M + 3) * SL;
-        const int j = 2 + kColG * g + q;
-        for (int k = c; k < n; k += kColCh) {
-            const int at = s_cbase[q] + k;
-            if (at < cap) d.runs_u[at] = make_int4(j, s_cf[q * 24 + k], s_cl[q * 24 + k], 0);
+        const int j = 64 * cb + lane;
+        for (int k = wv; k < n; k += 4) {
+            const int at = s_cbase[lane] + k;
+            if (at < cap) d.runs_u[at] = make_int4(j, s_cf[lane * 24 + k], s_cl[lane * 24 + k], 0);
         }
         return;
     }
     __shared__ int s_f[4][24], s_l[4][24], s_n[4], s_base[4];
-    int n = 0, line;
-    if (rows) {
-        const int i = 4 * bid + wv; /* rows 0..N+2 for counting, 2..N for runs */
-        line = i;
-        if (count_wet && i <= N + 2) {
-            int c = 0;
-            const int2 e = d.ext[i];
-            const int hi = e.y < M + 2 ? e.y : M + 2;
-            for (int j = e.x + lane; j <= hi; j += kWave) {
-                const int k = kh[i * W + j];
-                const int f = dry ? dry[i * W + j] : 0;
-                c += (k == 0 && !f);
-            }
-            for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
-            if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
-        }
-        if (i >= 2 && i <= N) {
-            int ns = scan_runs<true>(kh, W, i, M, SL, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv], d.ext[i],
-                                     dry);
-            n = ns < SL ? ns : SL;
-            if (lane == 0) {
-                d.moci[i] = n;
-                if (ns > SL) d.flags[0] = 1;
-            }
+    int n = 0;
+    const int i = 4 * bid + wv; /* rows 0..N+2 for counting, 2..N for runs */
+    if (count_wet && i <= N + 2) {
+        int c = 0;
+        for (int w = lane; w < ncb; w += kWave) c += __popcll(d.wetb[(size_t)i * ncb + w]);
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, kWave);
+        if (lane == 0 && c) atomicAdd((unsigned long long*)d.wet_count, (unsigned long long)c);
+    }
+    if (i >= 2 && i <= N) {
+        const int ns = row_word_scan(d, i, lane, d.daui + i * SL, d.cuoii + i * SL, s_f[wv], s_l[wv]);
+        n = ns < SL ? ns : SL;
+        if (lane == 0) {
+            d.moci[i] = n;
+            if (ns > SL) d.flags[0] = 1;
         }
     }
     if (lane == 0) s_n[wv] = n;
@@ -2401,11 +2415,23 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
     }
     __syncthreads();
     const int cap = (N + 3) * SL;
-    int4* table = d.runs_v;
     const int at = s_base[wv] + lane;
-    if (lane < n && at < cap) table[at] = make_int4(line, s_f[wv][lane], s_l[wv][lane], 0);
+    if (lane < n && at < cap) d.runs_v[at] = make_int4(i, s_f[wv][lane], s_l[wv][lane], 0);
 }
 
+/* The wet bits from khouot alone (no drying flags pending), for the segment
+ * finder after init and restarts: one ballot per 64-cell tile row. */
+__global__ void __launch_bounds__(256) k_wet_bits(const HdDev* __restrict__ dv)
+{
+    const HdDevG& d = dev_view(dv);
+    const int ncb = (d.M + 3 + 63) >> 6;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), i = blockIdx.y;
+    if (w >= ncb || i >= d.N + 3) return; /* whole waves */
+    const int j = 64 * w + (threadIdx.x & 63);
+    const bool wet = j < d.M + 3 && d.khouot[(size_t)i * d.W + j] == 0;
+    const unsigned long long bits = __ballot(wet);
+    if ((threadIdx.x & 63) == 0) d.wetb[(size_t)i * ncb + w] = bits;
+}
 
 __global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
 {
@@ -2508,6 +2534,7 @@ void hd_launch_preprocess(const HdBatch& b, hipStream_t s)
 
 void hd_launch_find_limits(const HdBatch& b, hipStream_t s)
 {
+    hipLaunchKernelGGL(k_wet_bits, dim3((b.M + 3 + 255) / 256, b.N + 3, b.n), dim3(256), 0, s, b.dv);
     hipLaunchKernelGGL(k_reset_runs, dim3(1, 1, b.n), dim3(64), 0, s, b.dv);
     hd_launch_segments(b, 0, s);
 }
@@ -2519,12 +2546,11 @@ bool hd_uvz_segments_fits(const HdBatch& b)
     return b.M - 1 <= 8 * 256 && imax <= 8 * 256;
 }
 
-/* table-scan blocks: rows 4 per block, columns kColG per block; counts
- * padded to whole rounds of the eight XCDs (idle blocks find no line) */
+/* table-scan blocks: rows 4 per block, columns 64 per block */
 static void scan_blocks(const HdBatch& b, int& nrowb, int& ncolb)
 {
-    nrowb = (((b.N + 3 + 3) / 4) + 7) & ~7;
-    ncolb = (((b.M - 1 + kColG - 1) / kColG) + 31) & ~31;
+    nrowb = (b.N + 3 + 3) / 4;
+    ncolb = (b.M + 3 + 63) / 64;
 }
 
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s)
