@@ -656,7 +656,7 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         x.swap_u();
         launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
     }
-    if (x.solver == 0 && hd_uvz_segments_fits(x.b())) {
+    if (x.solver == 0) {
         launch(x, "uvz_segments", [&] { hd_launch_uvz_segments(x.b(), o.kenhhepd, o.kenhhepng, ntiles, count_wet, s); });
     } else {
         launch(x, "uvz_dry_htd", [&] { hd_launch_uvz_dry(x.b(), o.kenhhepd, o.kenhhepng, x.bw, ntiles, s); });

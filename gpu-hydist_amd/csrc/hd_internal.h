@@ -120,7 +120,6 @@ struct HdBatch {
 /* kernel launchers (hd_kernels.hip, hd_sediment.hip), one kernel each */
 void hd_launch_onetime_init(const HdBatch& b, hipStream_t s);
 void hd_launch_find_limits(const HdBatch& b, hipStream_t s); /* wet bits from khouot, run-table reset, segments (+ FS edges) */
-bool hd_uvz_segments_fits(const HdBatch& b);
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s);
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s); /* also the FS edge interpolations */
 void hd_launch_htuongdoi(const HdBatch& b, hipStream_t s);

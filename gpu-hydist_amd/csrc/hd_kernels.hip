@@ -2016,38 +2016,135 @@ __global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, 
 
 /* ================================================================ segments */
 
-/* Interpolate_FS_ng, support_funcs.cu:189-208, one column j of the row
- * `location` (source row clamped to [0, N+1]; zero divisor -> quotient 0). */
-__device__ void fs_ng(const HdDevG& d, int location, int offset, int sign, int j)
+/* Interpolate_FS_ng (support_funcs.cu:189-208) of row N (p = 0) or row 2
+ * (p = 1) at column j, and Interpolate_FS_d (support_funcs.cu:210-229) of
+ * column M at row i: whether the cell moves (t > 0), its travel distance and
+ * its source row / column (clamped to [0, N+1] / [0, M+1]).  `dry` (the
+ * drying flags, when the state copy of the half-step runs in the same
+ * launch) gives the velocities that copy leaves: t zeroed at drying cells,
+ * u = t_u (1 - kd), v = t_v (1 - kn) (rows 0..N). */
+struct FsEdge {
+    bool on;
+    double dl;
+    int src;
+};
+
+__device__ __forceinline__ FsEdge fs_row_edge(const HdDevG& d, const unsigned char* dry, int kd, int p, int j)
 {
-    int W = d.W;
-    if (d.t_u[location * W + j] > 0) {
-        double deltaX = (d.u[location * W + j] + d.t_u[location * W + j]) * 0.25 * d.P.dT;
-        int i = (int)(offset + sign * (deltaX / d.P.dX));
-        if (i < 0) i = 0;
-        if (i > d.N + 1) i = d.N + 1;
-        int i_deltaX = hd_rint_int(deltaX);
-        int i_dX = hd_rint_int(d.P.dX);
-        int q = i_dX != 0 ? i_deltaX / i_dX : 0;
-        d.FS[location * W + j] = d.FS[i * W + j] + sign * (d.FS[(i + 1) * W + j] - d.FS[i * W + j]) * (deltaX / d.P.dX - q);
+    const int N = d.N, W = d.W, loc = p == 0 ? N : 2;
+    FsEdge e{false, 0.0, 0};
+    if (j < 1 || j >= d.M) return e;
+    double tq = d.t_u[loc * W + j], q = d.u[loc * W + j];
+    if (dry) {
+        if (dry[loc * W + j] || dry[(loc + 1) * W + j]) tq = 0;
+        q = tq * (1 - kd);
+    }
+    e.on = tq > 0;
+    e.dl = (q + tq) * 0.25 * d.P.dT;
+    const int offset = p == 0 ? N - 1 : 2, sign = p == 0 ? -1 : 1;
+    int r = (int)(offset + sign * (e.dl / d.P.dX));
+    if (r < 0) r = 0;
+    if (r > N + 1) r = N + 1;
+    e.src = r;
+    return e;
+}
+
+__device__ __forceinline__ FsEdge fs_col_edge(const HdDevG& d, const unsigned char* dry, int kn, int i)
+{
+    const int N = d.N, M = d.M, W = d.W;
+    FsEdge e{false, 0.0, 0};
+    double tq = d.t_v[i * W + M], q = d.v[i * W + M];
+    if (dry) {
+        if (dry[i * W + M] || dry[i * W + M + 1]) tq = 0;
+        if (i <= N) q = tq * (1 - kn);
+    }
+    e.on = tq > 0;
+    e.dl = (q + tq) * 0.25 * d.P.dT;
+    int r = (int)((M - 1) + -1 * (e.dl / d.P.dY));
+    if (r < 0) r = 0;
+    if (r > M + 1) r = M + 1;
+    e.src = r;
+    return e;
+}
+
+/* FS as the interpolations read it: zero where the state copy of the same
+ * launch dries the cell */
+__device__ __forceinline__ double fs_src(const HdDevG& d, const unsigned char* dry, int q)
+{
+    return (dry && dry[q]) ? 0.0 : d.FS[q];
+}
+
+__device__ __forceinline__ double fs_row_value(const HdDevG& d, int p, const FsEdge& e, double f0, double f1)
+{
+    const int i_dX = hd_rint_int(d.P.dX);
+    const int q = i_dX != 0 ? hd_rint_int(e.dl) / i_dX : 0;
+    const int sign = p == 0 ? -1 : 1;
+    return f0 + sign * (f1 - f0) * (e.dl / d.P.dX - q);
+}
+
+__device__ __forceinline__ double fs_col_value(const HdDevG& d, const FsEdge& e, double f0, double f1)
+{
+    const int j_dY = hd_rint_int(d.P.dY);
+    const int q = j_dY != 0 ? hd_rint_int(e.dl) / j_dY : 0;
+    return f1 + (f1 - f0) * (e.dl / d.P.dY - q);
+}
+
+/* The new values of rows N and 2 at column j (the reference runs row N,
+ * then row 2, then column M).  Each reads only its own column, so row 2
+ * sees row N's new value at (N, j) from the same thread, and row N reads
+ * (2, j) before this thread writes it. */
+__device__ __forceinline__ void fs_rows_at(const HdDevG& d, const unsigned char* dry, int kd, int j, bool store,
+                                           bool& onN, double& vN, bool& on2, double& v2)
+{
+    const int N = d.N, W = d.W;
+    const FsEdge e1 = fs_row_edge(d, dry, kd, 0, j), e2 = fs_row_edge(d, dry, kd, 1, j);
+    double a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    if (e1.on) { a0 = fs_src(d, dry, e1.src * W + j); a1 = fs_src(d, dry, (e1.src + 1) * W + j); }
+    if (e2.on) { b0 = fs_src(d, dry, e2.src * W + j); b1 = fs_src(d, dry, (e2.src + 1) * W + j); }
+    onN = e1.on;
+    on2 = e2.on;
+    if (e1.on) {
+        vN = fs_row_value(d, 0, e1, a0, a1);
+        if (store) d.FS[N * W + j] = vN;
+        if (e2.src == N) b0 = vN;
+        if (e2.src + 1 == N) b1 = vN;
+    }
+    if (e2.on) {
+        v2 = fs_row_value(d, 1, e2, b0, b1);
+        if (store) d.FS[2 * W + j] = v2;
     }
 }
 
-/* Interpolate_FS_d, support_funcs.cu:210-229, one row i of column `location`
- * (source column clamped to [0, M+1]). */
-__device__ void fs_d(const HdDevG& d, int location, int offset, int sign, int i)
+/* The FS edge interpolations, spread over workgroups of 256 without any
+ * ordering between them: workgroup k < nfsr takes columns 1 + 256k + t of
+ * rows N and 2; the others take rows i = 1 + 256 (k - nfsr) + t of column M.
+ * The column cell of row N (row 2) reads that row's new values at its two
+ * source columns, and computes them itself. */
+__device__ void fs_edges_block(const HdDevG& d, int k, int nfsr, const unsigned char* dry, int kd, int kn)
 {
-    int W = d.W;
-    if (d.t_v[i * W + location] > 0) {
-        double deltaY = (d.v[i * W + location] + d.t_v[i * W + location]) * 0.25 * d.P.dT;
-        int j = (int)(offset + sign * (deltaY / d.P.dY));
-        if (j < 0) j = 0;
-        if (j > d.M + 1) j = d.M + 1;
-        int j_deltaY = hd_rint_int(deltaY);
-        int j_dY = hd_rint_int(d.P.dY);
-        int q = j_dY != 0 ? j_deltaY / j_dY : 0;
-        d.FS[i * W + location] = d.FS[i * W + j + 1] + (d.FS[i * W + j + 1] - d.FS[i * W + j]) * (deltaY / d.P.dY - q);
+    const int N = d.N, M = d.M, W = d.W;
+    if (k < nfsr) {
+        const int j = 1 + 256 * k + (int)threadIdx.x;
+        bool onN, on2;
+        double vN, v2;
+        fs_rows_at(d, dry, kd, j, true, onN, vN, on2, v2);
+        return;
     }
+    const int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
+    const int i = 1 + 256 * (k - nfsr) + (int)threadIdx.x;
+    if (i > imax) return;
+    const FsEdge e = fs_col_edge(d, dry, kn, i);
+    if (!e.on) return;
+    double f0 = fs_src(d, dry, i * W + e.src), f1 = fs_src(d, dry, i * W + e.src + 1);
+    if (i == N || i == 2) {
+        bool onN, on2;
+        double vN, v2;
+        fs_rows_at(d, dry, kd, e.src, false, onN, vN, on2, v2);
+        if (i == N ? onN : on2) f0 = i == N ? vN : v2;
+        fs_rows_at(d, dry, kd, e.src + 1, false, onN, vN, on2, v2);
+        if (i == N ? onN : on2) f1 = i == N ? vN : v2;
+    }
+    d.FS[i * W + M] = fs_col_value(d, e, f0, f1);
 }
 
 __global__ void k_reset_runs(const HdDev* __restrict__ dv)
@@ -2136,101 +2233,6 @@ __device__ __forceinline__ int row_word_scan(const HdDevG& d, int i, int lane, i
         ne += __shfl(pe, kWave - 1, kWave);
     }
     return ns;
-}
-
-/* Interpolate_FS_ng rows N and 2, then Interpolate_FS_d column M, by one
- * workgroup (the two reference kernels' order; the column reads the rows'
- * corner results).  Each edge cell touches only its own FS entry and cells
- * of its own line, so the cells of one edge are independent.  The velocity
- * loads of all three edges go out together; per edge the FS loads follow
- * (one round trip), with a barrier between the edges. */
-__device__ void fs_edges_block(const HdDevG& d, const unsigned char* dry, int kd, int kn)
-{
-    const int N = d.N, M = d.M, W = d.W;
-    const int imax = (M - 1 < N + 2) ? M - 1 : N + 2;
-    const int T = blockDim.x, tid = threadIdx.x;
-    constexpr int K = 8;
-    if (!dry && (M - 1 > K * T || imax > K * T)) { /* very wide grid: edge by edge */
-        for (int j = 1 + tid; j < M; j += T) fs_ng(d, N, N - 1, -1, j);
-        __syncthreads();
-        for (int j = 1 + tid; j < M; j += T) fs_ng(d, 2, 2, 1, j);
-        __syncthreads();
-        for (int i = 1 + tid; i <= imax; i += T) fs_d(d, M, M - 1, -1, i);
-        return;
-    }
-    const double dT = d.P.dT, dX = d.P.dX, dY = d.P.dY;
-    double dl[3][K];
-    bool on[3][K];
-#pragma unroll
-    for (int p = 0; p < 3; ++p) {
-#pragma unroll
-        for (int b = 0; b < K; ++b) {
-            const int x = 1 + tid + b * T;
-            double tq = 0, q = 0;
-            bool valid;
-            if (p < 2) {
-                const int loc = p == 0 ? N : 2;
-                valid = x < M;
-                if (valid) {
-                    tq = d.t_u[loc * W + x];
-                    q = d.u[loc * W + x];
-                    if (dry) { /* the values the state copy leaves (k_uvz) */
-                        if (dry[loc * W + x] || dry[(loc + 1) * W + x]) tq = 0;
-                        q = tq * (1 - kd);
-                    }
-                }
-            } else {
-                valid = x <= imax;
-                if (valid) {
-                    tq = d.t_v[x * W + M];
-                    q = d.v[x * W + M];
-                    if (dry) {
-                        if (dry[x * W + M] || dry[x * W + M + 1]) tq = 0;
-                        if (x <= N) q = tq * (1 - kn);
-                    }
-                }
-            }
-            on[p][b] = valid && tq > 0;
-            dl[p][b] = (q + tq) * 0.25 * dT;
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < 2; ++p) { /* Interpolate_FS_ng, support_funcs.cu:189-208 */
-        const int location = p == 0 ? N : 2, offset = p == 0 ? N - 1 : 2, sign = p == 0 ? -1 : 1;
-        const int i_dX = hd_rint_int(dX);
-#pragma unroll
-        for (int b = 0; b < K; ++b) {
-            if (!on[p][b]) continue;
-            const int j = 1 + tid + b * T;
-            const double deltaX = dl[p][b];
-            int i = (int)(offset + sign * (deltaX / dX));
-            if (i < 0) i = 0;
-            if (i > N + 1) i = N + 1;
-            const int i_deltaX = hd_rint_int(deltaX);
-            const int q = i_dX != 0 ? i_deltaX / i_dX : 0;
-            double f0 = d.FS[i * W + j], f1 = d.FS[(i + 1) * W + j];
-            if (dry) { f0 = dry[i * W + j] ? 0.0 : f0; f1 = dry[(i + 1) * W + j] ? 0.0 : f1; }
-            d.FS[location * W + j] = f0 + sign * (f1 - f0) * (deltaX / dX - q);
-        }
-        __syncthreads();
-    }
-    {   /* Interpolate_FS_d, support_funcs.cu:210-229, column M */
-        const int j_dY = hd_rint_int(dY);
-#pragma unroll
-        for (int b = 0; b < K; ++b) {
-            if (!on[2][b]) continue;
-            const int i = 1 + tid + b * T;
-            const double deltaY = dl[2][b];
-            int j = (int)((M - 1) + -1 * (deltaY / dY));
-            if (j < 0) j = 0;
-            if (j > M + 1) j = M + 1;
-            const int j_deltaY = hd_rint_int(deltaY);
-            const int q = j_dY != 0 ? j_deltaY / j_dY : 0;
-            double f0 = d.FS[i * W + j], f1 = d.FS[i * W + j + 1];
-            if (dry) { f0 = dry[i * W + j] ? 0.0 : f0; f1 = dry[i * W + j + 1] ? 0.0 : f1; }
-            d.FS[i * W + M] = f1 + (f1 - f0) * (deltaY / dY - q);
-        }
-    }
 }
 
 /* Transpose of a 64 x 64 bit block held one row per lane (bit q of lane r
@@ -2352,14 +2354,15 @@ __device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, 
  * 4b .. 4b+3, one wave per row; with count_wet these waves also add the
  * row's wet cells (rows 0..N+2) to the wet-cell accumulator.  Blocks
  * [nrowb, nrowb + ncolb): the 64 columns of word column b - nrowb
- * (col_word_scan).  Block nrowb + ncolb: the FS edge interpolations; `dry`
- * (the drying flags, when the state copy runs in the same launch) marks
- * the cells that copy zeroes. */
-__device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int count_wet,
-                                                  const unsigned char* dry, int kd, int kn)
+ * (col_word_scan).  The next nfs blocks: the FS edge interpolations
+ * (fs_edges_block, nfsr of them for the rows); `dry` (the drying flags, when
+ * the state copy runs in the same launch) marks the cells that copy
+ * zeroes. */
+__device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int nrowb, int ncolb, int nfsr,
+                                                  int count_wet, const unsigned char* dry, int kd, int kn)
 {
-    if (bid == nrowb + ncolb) { /* the FS edge interpolations (independent of the mask) */
-        fs_edges_block(d, dry, kd, kn);
+    if (bid >= nrowb + ncolb) { /* the FS edge interpolations (independent of the mask) */
+        fs_edges_block(d, bid - nrowb - ncolb, nfsr, dry, kd, kn);
         return;
     }
     const int N = d.N, M = d.M, SL = d.SL;
@@ -2368,6 +2371,7 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
     if (bid >= nrowb) { /* 64 columns per block */
         __shared__ int s_cf[kWave * 24], s_cl[kWave * 24], s_cbase[kWave];
         const int cb = bid - nrowb;
+        if (64 * cb > M) return; /* no column of this context (a smaller member of a batch) */
         const int n = col_word_scan(d, cb, s_cf, s_cl);
         if (wv == 0) { /* slots of the block's columns in column order, one atomic */
             int p = n;
@@ -2433,27 +2437,28 @@ __global__ void __launch_bounds__(256) k_wet_bits(const HdDev* __restrict__ dv)
     if ((threadIdx.x & 63) == 0) d.wetb[(size_t)i * ncb + w] = bits;
 }
 
-__global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int count_wet)
+__global__ void __launch_bounds__(256) k_find_limits(const HdDev* __restrict__ dv, int nrowb, int ncolb, int nfsr,
+                                                      int count_wet)
 {
-    find_limits_block(dev_view(dv), blockIdx.x, nrowb, ncolb, count_wet, nullptr, 0, 0);
+    find_limits_block(dev_view(dv), blockIdx.x, nrowb, ncolb, nfsr, count_wet, nullptr, 0, 0);
 }
 
 /* The state copy (k_uvz) and the segment tables + FS edges (k_find_limits)
- * of one half-step in one launch: blocks [0, nrowb + ncolb] build the
- * tables, the rest copy the state over the active row tiles (or the whole
- * grid).  The table blocks do not wait for the copy: a cell is wet iff
- * khouot == 0 and its drying flag is clear (the copy sets khouot = 1 exactly
- * at flagged cells), and the FS edge block applies the copy's zeroing and
- * u = t_u (1 - kd), v = t_v (1 - kn) to what it reads, so either order of
- * the two halves gives the result of copying first.  The run counters were
- * cleared by the smoothing pass before the wetting kernel. */
+ * of one half-step in one launch: blocks [0, nseg) build the tables and
+ * interpolate the FS edges, the rest copy the state over the active row
+ * tiles (or the whole grid).  The table blocks do not wait for the copy:
+ * the wet bits already exclude the cells it dries (it sets khouot = 1
+ * exactly at flagged cells), and the FS edge blocks apply the copy's
+ * zeroing and u = t_u (1 - kd), v = t_v (1 - kn) to what they read, so
+ * either order of the two halves gives the result of copying first.  The
+ * run counters were cleared by the wetting kernel. */
 __global__ void __launch_bounds__(256) k_uvz_segments(const HdDev* __restrict__ dv, int kd, int kn, int tiled,
-                                                       int nrowb, int ncolb, int count_wet)
+                                                       int nrowb, int ncolb, int nfsr, int nfs, int count_wet)
 {
     const HdDevG& d = dev_view(dv);
-    const int nseg = nrowb + ncolb + 1;
+    const int nseg = nrowb + ncolb + nfs;
     if ((int)blockIdx.x < nseg) {
-        find_limits_block(d, blockIdx.x, nrowb, ncolb, count_wet, d.dryflag, kd, kn);
+        find_limits_block(d, blockIdx.x, nrowb, ncolb, nfsr, count_wet, d.dryflag, kd, kn);
         return;
     }
     const int t = ((int)blockIdx.x - nseg) * 4 + (threadIdx.x >> 6);
@@ -2539,36 +2544,33 @@ void hd_launch_find_limits(const HdBatch& b, hipStream_t s)
     hd_launch_segments(b, 0, s);
 }
 
-bool hd_uvz_segments_fits(const HdBatch& b)
-{
-    /* the fused kernel's FS edge block holds up to 8 cells per thread */
-    const int imax = (b.M - 1 < b.N + 2) ? b.M - 1 : b.N + 2;
-    return b.M - 1 <= 8 * 256 && imax <= 8 * 256;
-}
-
-/* table-scan blocks: rows 4 per block, columns 64 per block */
-static void scan_blocks(const HdBatch& b, int& nrowb, int& ncolb)
+/* table-scan blocks: rows 4 per block, columns 64 per block; FS edge
+ * blocks: nfsr for the columns of rows N and 2, then the rows of column M */
+static void scan_blocks(const HdBatch& b, int& nrowb, int& ncolb, int& nfsr, int& nfs)
 {
     nrowb = (b.N + 3 + 3) / 4;
     ncolb = (b.M + 3 + 63) / 64;
+    const int imax = (b.M - 1 < b.N + 2) ? b.M - 1 : b.N + 2;
+    nfsr = (b.M - 1 + 255) / 256;
+    nfs = nfsr + (imax + 255) / 256;
 }
 
 void hd_launch_uvz_segments(const HdBatch& b, int kd, int kn, int ntiles, int count, hipStream_t s)
 {
-    int nrowb, ncolb;
-    scan_blocks(b, nrowb, ncolb);
+    int nrowb, ncolb, nfsr, nfs;
+    scan_blocks(b, nrowb, ncolb, nfsr, nfs);
     int ncb = (b.M + 3 + 63) / 64;
     int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ncb;
-    hipLaunchKernelGGL(k_uvz_segments, dim3(nrowb + ncolb + 1 + (tiles + 3) / 4, 1, b.n), dim3(256), 0, s, b.dv, kd,
-                       kn, ntiles > 0 ? 1 : 0, nrowb, ncolb, count);
+    hipLaunchKernelGGL(k_uvz_segments, dim3(nrowb + ncolb + nfs + (tiles + 3) / 4, 1, b.n), dim3(256), 0, s, b.dv,
+                       kd, kn, ntiles > 0 ? 1 : 0, nrowb, ncolb, nfsr, nfs, count);
 }
 
 void hd_launch_segments(const HdBatch& b, int count, hipStream_t s)
 {
-    int nrowb, ncolb;
-    scan_blocks(b, nrowb, ncolb);
-    /* + one block for the FS edge interpolations */
-    hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb + 1, 1, b.n), dim3(256), 0, s, b.dv, nrowb, ncolb, count);
+    int nrowb, ncolb, nfsr, nfs;
+    scan_blocks(b, nrowb, ncolb, nfsr, nfs);
+    hipLaunchKernelGGL(k_find_limits, dim3(nrowb + ncolb + nfs, 1, b.n), dim3(256), 0, s, b.dv, nrowb, ncolb, nfsr,
+                       count);
 }
 
 
