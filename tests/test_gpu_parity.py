@@ -96,6 +96,46 @@ def test_sediment_starting_mid_run_bitwise():
     assert_same_state(o, g, "sediment from step 13, bed change from step 25")
 
 
+def test_fs_edges_reading_new_rows_bitwise():
+    """The FS edge interpolations with velocities that make row 2 read row N
+    (after row N was interpolated), row N read row 2 (before row 2 was), and
+    the column-M cells of rows N and 2 read those rows' new values: the
+    reference's kernel order (row N, row 2, column M) against the checker's
+    segment finder, through hd_prepare."""
+    case = synth.channel_case(N=16, M=40, nsteps=10, name="channel_16x40")  # column M spans rows 1..N+2
+    o, g = pair(case)
+    N, M = case.N, case.M
+    P = g.params()
+    rng = np.random.default_rng(11)
+    fs = rng.uniform(0.0, 1.0, (N + 3, M + 3))
+    t_u = g.get("t_u").reshape(N + 3, M + 3).copy()
+    t_v = g.get("t_v").reshape(N + 3, M + 3).copy()
+
+    def speed(cells_per_step, dx):
+        # u = t_u = s: travel (u + t_u) dT / 4 = cells_per_step * dx
+        return 2.0 * cells_per_step * dx / P.dT
+
+    j = np.arange(1, M)
+    # row 2: sources N - 1 and N on alternate columns (reads row N)
+    t_u[2, 1:M] = np.where(j % 2 == 0, speed(N - 2.5, P.dX), speed(N - 1.5, P.dX))
+    # row N: sources 2 (reads row 2) on every third column, one cell back elsewhere
+    t_u[N, 1:M] = np.where(j % 3 == 0, speed(N - 3.5, P.dX), speed(0.5, P.dX))
+    t_u[N, 4] = -1.0  # a column where row N does not move
+    # column M, rows N and 2: sources M - 2 (reads the rows' columns M-2, M-1)
+    t_v[:, M] = speed(0.25, P.dY)
+    t_v[N, M] = t_v[2, M] = speed(0.5, P.dY)
+    src2 = ((2 + (t_u[2, 1:M] * 0.5 * P.dT) / P.dX).astype(int))
+    assert {N - 1, N} <= set(src2.tolist())
+    for name, arr in (("FS", fs), ("t_u", t_u), ("u", t_u), ("t_v", t_v), ("v", t_v)):
+        o.set(name, arr)
+        g.set(name, arr)
+    o.find_limits()
+    g.prepare()
+    assert np.array_equal(o.get("FS"), g.get("FS"))
+    for name in ("daui", "cuoii", "moci", "dauj", "cuoij", "mocj"):
+        assert np.array_equal(o.get(name), g.get(name)), name
+
+
 def test_bed_change_reference_mode():
     case = small_river_sediment(qb_mode=QB_REFERENCE)
     o, g = pair(case)
