@@ -605,11 +605,10 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, "adi_spec_uz", [&] { hd_launch_adi_spec(x.b(), 1, shift, x.run_blocks_u, s); });
         }
         if (x.solver == 0) {
-            /* smoothing of t_u and the cells of solveV by row tiles in one
-             * launch, then the margins per row */
+            /* smoothing of t_u, the cells of solveV and the rows' margins by
+             * row tiles in one launch */
             launch(x, "smooth_u_explicit_v", [&] { hd_launch_smooth_u_explicit_v(x.b(), x.bw, ntiles, s); });
             x.swap_u();
-            launch(x, "margins_v", [&] { hd_launch_explicit_margins(x.b(), 0, 0, s); });
         } else {
             launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
             x.swap_u();
@@ -637,11 +636,10 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
             launch(x, "adi_spec_vz", [&] { hd_launch_adi_spec(x.b(), 0, shift, x.run_blocks_v, s); });
         }
         if (x.solver == 0) {
-            /* smoothing of t_v and the cells of solveU by row tiles in one
-             * launch, then the margins per column */
+            /* smoothing of t_v, the cells of solveU and the columns' margins
+             * by row tiles in one launch */
             launch(x, "smooth_v_explicit_u", [&] { hd_launch_smooth_v_explicit_u(x.b(), x.bw, ntiles, s); });
             x.swap_v();
-            launch(x, "margins_u", [&] { hd_launch_explicit_margins(x.b(), 1, 0, s); });
         } else {
             launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
             x.swap_v();

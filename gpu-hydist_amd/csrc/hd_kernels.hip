@@ -1759,12 +1759,99 @@ __device__ __forceinline__ double smooth_u_at(const HdDevG& d, int qi, int qj)
     return r;
 }
 
+/* solveV's t_v at (i, x) / solveU's t_u at (x, j) as the fused kernels
+ * below compute it: vSolver / uSolver on the other direction's smoothed
+ * field at the four faces around the cell. */
+__device__ __forceinline__ double explicit_v_at(const HdDevG& d, int i, int x)
+{
+    ExRaw r = v_solver_ld(d, i, x);
+    r.t0 = smooth_u_at(d, i - 1, x);
+    r.t1 = smooth_u_at(d, i, x);
+    r.t2 = smooth_u_at(d, i - 1, x + 1);
+    r.t3 = smooth_u_at(d, i, x + 1);
+    return v_solver_ev(d.P, r);
+}
+
+__device__ __forceinline__ double explicit_u_at(const HdDevG& d, int x, int j)
+{
+    ExRaw r = u_solver_ld(d, x, j);
+    r.t0 = smooth_v_at(d, x, j - 1);
+    r.t1 = smooth_v_at(d, x, j);
+    r.t2 = smooth_v_at(d, x + 1, j - 1);
+    r.t3 = smooth_v_at(d, x + 1, j);
+    return u_solver_ev(d.P, r);
+}
+
+/* update_margin_elem_V (UVZSolver_multithread.cu:1008-1052; ROWS: the runs
+ * of row `line`, positions = columns, L = M) / update_margin_elem_U
+ * (:970-1006; the runs of column `line`, positions = rows, L = N) at
+ * position x, folded into the explicit pass.  The runs are those of the
+ * segment tables, recognised from the mask they were built from (khouot
+ * changes only later in the half-step); wx, wn, wl2, wl1 are the wetness of
+ * positions x, x+1, L-2 and L-1 of the line.  A run [first, last] has
+ * margins first-1 and last, both 0 except at an open grid edge: first == 2
+ * with the depth at first-1 not at the wall value gives 2 E(2) - E(3),
+ * last == L likewise 2 E(L-1) - E(L-2), E being the line after solveU /
+ * solveV.  E(L-1), E(L-2) are explicit cells of the run, computed here;
+ * E(2), E(3) are too unless the run is [2, 2] or [2, 3], where the value
+ * before the margins enters.  Then the run's margin last (and, after
+ * [2, 2], the margin 3 of a run starting at 4) is written by position 1's
+ * thread after it read them, in the reference's order, and not by their own
+ * threads.  Returns the number of margin values this thread writes (0..3),
+ * at positions pos[k] with values val[k]; only positions 1..3 and L load
+ * anything. */
+template <bool ROWS>
+__device__ __forceinline__ int margin_at(const HdDevG& d, int line, int x, bool wx, bool wn, bool wl2, bool wl1,
+                                         int pos[3], double val[3])
+{
+    const int N = d.N, M = d.M, W = d.W;
+    const int L = ROWS ? M : N;
+    if (x < 1 || x > L) return 0;
+    const bool first_m = wn && !wx && x + 1 <= L - 2;
+    const bool last_m = wx && !wn && (x <= L - 2 || (x == L - 1 && wl2) || (x == L && wl1 && wl2));
+    if (!first_m && !last_m) return 0;
+    pos[0] = x;
+    val[0] = 0.0;
+    if (x > 3 && x < L) return 1; /* first > 4, or last < L: a wall margin */
+    const int* kh = d.khouot;
+    const double* h = d.h;
+    const double* t = ROWS ? d.t_v : d.t_u;
+    auto at = [&](int k) { return ROWS ? line * W + k : k * W + line; };
+    auto wet = [&](int k) -> bool { return k >= 2 && k <= L && kh[at(k)] == 0; };
+    auto ex_at = [&](int k) -> double { return ROWS ? explicit_v_at(d, line, k) : explicit_u_at(d, k, line); };
+    const double eps = 1e-16; /* UVZSolver_multithread.cu:12 */
+    const double ND = d.P.NANGDAY;
+    if (x == L) {
+        const bool wall2 = ROWS ? fabs((h[line * W + M] + h[(line - 1) * W + M]) * 0.5 - ND) < eps
+                                : fabs((h[N * W + line] + h[N * W + line - 1]) * 0.5 - ND) < eps;
+        if (!wall2) val[0] = 2 * ex_at(L - 1) - ex_at(L - 2);
+        return 1;
+    }
+    /* x <= 3: the run from 2, if any -- does it end at 2 or 3, does its
+     * first margin extrapolate? */
+    const bool wall1 = ROWS ? fabs((h[line * W + 1] + h[(line - 1) * W + 1]) * 0.5 - ND) < eps
+                            : (fabs(h[1 * W + line] + h[1 * W + line - 1]) * 0.5 - ND) < eps;
+    const bool w2 = wet(2), w3 = wet(3), w4 = wet(4);
+    const bool end2 = w2 && !w3, end3 = w2 && w3 && !w4;
+    if (w2 && !wall1 && (end2 || end3)) {
+        if (x != 1) return 0;
+        const double e2 = end2 ? t[at(2)] : ex_at(2);
+        val[0] = 2 * e2 - t[at(3)];
+        pos[1] = end2 ? 2 : 3;
+        val[1] = 0.0;
+        if (end2 && w4 && 4 <= L - 2) { pos[2] = 3; val[2] = 0.0; return 3; }
+        return 2;
+    }
+    if (x == 1 && !wall1) val[0] = 2 * ex_at(2) - ex_at(3);
+    return 1;
+}
+
 /* The UZ half-step's smoothing of t_u and solveV (UVZSolver_multithread.cu:
  * 1055-1071) in one pass over the active row tiles: the cells first ..
  * last-1 of every wet run of every row, recognised from the mask as in
  * k_explicit_rows_u (k and k+1 wet, and for k = M-1 also M-2), with the
  * smoothed t_u at the four faces around the cell recomputed from the
- * unsmoothed buffer.  k_explicit_margins (cells off) writes the margins. */
+ * unsmoothed buffer, and the margins of the rows' runs (margin_at). */
 __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled)
 {
     const HdDevG& d = dev_view(dv);
@@ -1773,17 +1860,35 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
     const int N = d.N, M = d.M, W = d.W;
     const int p = i * W + j;
     const double sm = smooth_u_at(d, i, j);
-    d.tmp_u[p] = sm;
-    if (i < 2 || i > N || j < 2 || j > M - 1) return;
+    /* the row's wetness at j, j+1, M-2 and M-1: solveV's cells and the
+     * runs' margins */
     const int* kh = d.khouot;
-    const bool in = kh[p] == 0 && kh[p + 1] == 0 && (j <= M - 2 || kh[i * W + M - 2] == 0);
-    if (!in) return;
-    ExRaw r = v_solver_ld(d, i, j);
-    r.t0 = smooth_u_at(d, i - 1, j);
-    r.t1 = sm;
-    r.t2 = smooth_u_at(d, i - 1, j + 1);
-    r.t3 = smooth_u_at(d, i, j + 1);
-    d.t_v[p] = v_solver_ev(d.P, r);
+    const bool rowok = i >= 2 && i <= N;
+    bool wx = false, wn = false, wl2 = false, wl1 = false;
+    if (rowok) {
+        wx = j >= 2 && j <= M && kh[p] == 0;
+        wn = j + 1 >= 2 && j + 1 <= M && kh[p + 1] == 0;
+        wl2 = kh[i * W + M - 2] == 0;
+        wl1 = kh[i * W + M - 1] == 0;
+    }
+    const bool in = rowok && j >= 2 && j <= M - 1 && wx && wn && (j <= M - 2 || wl2);
+    double vn = 0;
+    if (in) {
+        ExRaw r = v_solver_ld(d, i, j);
+        r.t0 = smooth_u_at(d, i - 1, j);
+        r.t1 = sm;
+        r.t2 = smooth_u_at(d, i - 1, j + 1);
+        r.t3 = smooth_u_at(d, i, j + 1);
+        vn = v_solver_ev(d.P, r);
+    }
+    d.tmp_u[p] = sm;
+    if (in) d.t_v[p] = vn;
+    if (rowok) {
+        int mp[3];
+        double mv[3];
+        const int nm = margin_at<true>(d, i, j, wx, wn, wl2, wl1, mp, mv);
+        for (int k = 0; k < nm; ++k) d.t_v[i * W + mp[k]] = mv[k];
+    }
 }
 
 /* The VZ half-step's smoothing of t_v (Normalize + update_buffer into the
@@ -1791,7 +1896,8 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
  * over the active row tiles: solveU reads the smoothed t_v at the four faces
  * around its cell, which this thread recomputes from the unsmoothed buffer
  * (the values the smoothing pass writes), so the two need no launch
- * boundary between them. */
+ * boundary between them; the margins of the columns' runs (margin_at)
+ * too. */
 __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled)
 {
     const HdDevG& d = dev_view(dv);
@@ -1800,17 +1906,35 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
     const int N = d.N, M = d.M, W = d.W;
     const int p = i * W + j;
     const double sm = smooth_v_at(d, i, j);
-    d.tmp_v[p] = sm;
-    if (j < 2 || j > M || i < 2 || i > N - 1) return;
+    /* the column's wetness at i, i+1, N-2 and N-1: solveU's cells and the
+     * runs' margins */
     const int* kh = d.khouot;
-    const bool in = kh[p] == 0 && kh[p + W] == 0 && (i <= N - 2 || kh[(N - 2) * W + j] == 0);
-    if (!in) return;
-    ExRaw r = u_solver_ld(d, i, j);
-    r.t0 = smooth_v_at(d, i, j - 1);
-    r.t1 = sm;
-    r.t2 = smooth_v_at(d, i + 1, j - 1);
-    r.t3 = smooth_v_at(d, i + 1, j);
-    d.t_u[p] = u_solver_ev(d.P, r);
+    const bool colok = j >= 2 && j <= M;
+    bool wx = false, wn = false, wl2 = false, wl1 = false;
+    if (colok) {
+        wx = i >= 2 && i <= N && kh[p] == 0;
+        wn = i + 1 >= 2 && i + 1 <= N && kh[p + W] == 0;
+        wl2 = kh[(N - 2) * W + j] == 0;
+        wl1 = kh[(N - 1) * W + j] == 0;
+    }
+    const bool in = colok && i >= 2 && i <= N - 1 && wx && wn && (i <= N - 2 || wl2);
+    double un = 0;
+    if (in) {
+        ExRaw r = u_solver_ld(d, i, j);
+        r.t0 = smooth_v_at(d, i, j - 1);
+        r.t1 = sm;
+        r.t2 = smooth_v_at(d, i + 1, j - 1);
+        r.t3 = smooth_v_at(d, i + 1, j);
+        un = u_solver_ev(d.P, r);
+    }
+    d.tmp_v[p] = sm;
+    if (in) d.t_u[p] = un;
+    if (colok) {
+        int mp[3];
+        double mv[3];
+        const int nm = margin_at<false>(d, j, i, wx, wn, wl2, wl1, mp, mv);
+        for (int k = 0; k < nm; ++k) d.t_u[mp[k] * W + j] = mv[k];
+    }
 }
 
 /* ================================================================ wet / dry */
