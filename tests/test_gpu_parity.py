@@ -136,6 +136,38 @@ def test_fs_edges_reading_new_rows_bitwise():
         assert np.array_equal(o.get(name), g.get(name)), name
 
 
+def short_edge_runs_case():
+    """A channel with dry cells (3, 8) and (4, 12): columns 8 and 12 start
+    with the runs [2, 2] (followed by a run from 4) and [2, 3] at the open
+    inflow edge, whose margins the position-1 cell writes in order."""
+    case = synth.channel_case(N=24, M=30, nsteps=40, name="short_edge_runs")
+    h = case.arrays["h"].copy()
+    h[2:4, 7:9] = 0.0
+    h[3:5, 11:13] = 0.0
+    case.arrays["h"] = synth.remove_pinches(h)
+    return case
+
+
+def test_short_runs_at_open_edge_bitwise():
+    """Margins of runs [2, 2] and [2, 3] at an open edge (2 E(2) - E(3) with
+    the old values of the run's own last cell and of the next run's first
+    margin) match the checker step by step."""
+    case = short_edge_runs_case()
+    o, g = pair(case)
+    SL = o.params().segment_limit
+    mocj, dauj, cuoij = o.get("mocj"), o.get("dauj").reshape(-1, SL), o.get("cuoij").reshape(-1, SL)
+    assert mocj[8] >= 2 and (dauj[8, 0], cuoij[8, 0], dauj[8, 1]) == (2, 2, 4)
+    assert mocj[12] >= 1 and (dauj[12, 0], cuoij[12, 0]) == (2, 3)
+    op = case.opts()
+    t = 0.0
+    for k in range(30):
+        tn = o.step(t, op)
+        g.step(t, op)
+        t = tn
+        if k % 10 == 9:
+            assert_same_state(o, g, f"short edge runs, step {k + 1}")
+
+
 def test_bed_change_reference_mode():
     case = small_river_sediment(qb_mode=QB_REFERENCE)
     o, g = pair(case)
