@@ -1980,14 +1980,17 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
     const double *z = d.z, *ht = d.htaiz;
     int* kh = d.khouot;
     auto at = [&](int k) -> int { return horizontal ? line * W + k : k * W + line; };
-    auto open = [&](int q, int src) {
-        int pq = at(q), ps = at(src);
-        tz[pq] = tz[ps];
-        Hm[pq] = ht[pq] + tz[ps];
+    /* open cell q from a source with level tzs and concentration fss (the
+     * reference copies t_z and FS from the source, H_moi = htaiz + t_z) */
+    auto open = [&](int q, double tzs, double fss, double htq) {
+        const int pq = at(q);
+        const double hm = htq + tzs;
+        tz[pq] = tzs;
+        Hm[pq] = hm;
         kh[pq] = 0;
-        FS[pq] = FS[ps];
+        FS[pq] = fss;
         bool inrange = horizontal ? (q >= 2 && q <= M && line >= 1 && line <= N) : (q >= 2 && q <= N && line >= 1 && line <= M);
-        const bool dr = inrange && (Hm[pq] <= d.P.H_TINH);
+        const bool dr = inrange && (hm <= d.P.H_TINH);
         dryflag[pq] = dr;
         if (!dr) { /* the opened cell counts as wet for the segment finder */
             const int r = horizontal ? line : q, cc = horizontal ? q : line;
@@ -1995,9 +1998,12 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
         }
     };
     /* visit positions in scan order: the recorded ones, and after a cell
-     * opens its right neighbour, that neighbour (the reference scan's cascade) */
+     * opens its right neighbour, that neighbour (the reference scan's
+     * cascade), which carries the source's t_z and FS.  Everything a visit
+     * reads goes out in one round of loads, before its stores. */
     int li = 0, pos = 1;
     bool cascade = false;
+    double tzc = 0, fsc = 0;
     while (true) {
         if (cascade) {
             ++pos;
@@ -2009,11 +2015,21 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
             pos = list[li];
         }
         if (pos < 2 || pos > pmax) break;
+        const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
+        const double zc = z[pp];
+        const double tzv = cascade ? tzc : tz[pp];
+        const double fsv = cascade ? fsc : FS[pp];
+        const int km = kh[pm], kn = kh[pn];
+        const double htm = ht[pm], htn = ht[pn];
         cascade = false;
-        int pp = at(pos);
-        if (tz[pp] > z[pp]) {
-            if (kh[at(pos - 1)] == 1) open(pos - 1, pos);
-            if (kh[at(pos + 1)] == 1) { open(pos + 1, pos); cascade = true; }
+        if (tzv > zc) {
+            if (km == 1) open(pos - 1, tzv, fsv, htm);
+            if (kn == 1) {
+                open(pos + 1, tzv, fsv, htn);
+                cascade = true;
+                tzc = tzv;
+                fsc = fsv;
+            }
         }
     }
 }
