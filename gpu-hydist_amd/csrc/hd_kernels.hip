@@ -1940,45 +1940,41 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
 /* ================================================================ wet / dry */
 
 /* Wetting pass of Reset_states_horizontal (support_funcs.cu:92-112) /
- * _vertical (:152-171), one lane per line.  The serial scan can only act at
- * positions next to a khouot == 1 cell; those were recorded by k_normalize
- * and are visited here in increasing order, re-checking the current state, so
- * cascades along the line happen exactly as in the serial loop.  A line with
- * more candidates than fit the list is scanned in full. */
-__global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
+ * _vertical (:152-171), one wave per line.  The serial scan can only act at
+ * positions next to a khouot == 1 cell; those were recorded by k_normalize.
+ * The wave sorts them (ranks by comparison), every lane loads the window of
+ * one recorded position p at once (khouot and htaiz at p-1 .. p+kWetLA+1, z
+ * at p .. p+kWetLA, t_z and FS at p), and lane 0 replays the scan over the
+ * recorded positions in increasing order from LDS: a visit at q opens q-1
+ * and q+1 where khouot == 1 and t_z(q) > z(q), copying q's t_z and FS;
+ * opening q+1 makes q+1 the next visit, with q's values (the reference's
+ * cascade).  Within a window every value is current except khouot at q-1
+ * when an earlier visit opened it from the left, which is tracked; a
+ * cascade past the window reads global memory.  A line with one or two
+ * candidates is visited directly by lane 0, one with more candidates than
+ * the list holds scanned in full. */
+constexpr int kWetLA = 3;           /* cascade steps a window covers */
+constexpr int kWetFew = 2;          /* up to this many candidates lane 0 visits them directly */
+constexpr int kWetWin = kWetLA + 3; /* khouot / htaiz entries p-1 .. p+kWetLA+1 */
+
+__global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal)
 {
     const HdDevG& d = dev_view(dv);
-    int* cand_cnt = d.cand_cnt;
-    int* cand = d.cand;
-    unsigned char* dryflag = d.dryflag;
     /* the run tables are consumed for this half-step (solver and sediment
      * sweeps); the segment finder at its end appends from zero */
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
-    /* each lane's candidate list lives in LDS (a private array would sit in
-     * scratch memory, one global round trip per sort step) */
-    __shared__ int s_list[64][kCand + 1];
-    int* list = s_list[threadIdx.x];
-    int line = blockIdx.x * blockDim.x + threadIdx.x;
-    int N = d.N, M = d.M, W = d.W;
-    int lo = horizontal ? 1 : 1, hi = horizontal ? N : M;
-    if (line < lo || line > hi) return;
-    int n = cand_cnt[line];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+    const int lane = threadIdx.x, line = blockIdx.x;
+    const int N = d.N, M = d.M, W = d.W;
+    if (line < 1 || line > (horizontal ? N : M)) return;
+    int* cand_cnt = d.cand_cnt;
+    const int n = cand_cnt[line];
     if (n == 0) return;
-    cand_cnt[line] = 0;
-    bool full = n > kCand;
-    int cnt = full ? 0 : n;
-    if (!full) {
-        for (int t = 0; t < n; ++t) list[t] = cand[line * kCand + t];
-        for (int a = 1; a < n; ++a) { /* insertion sort */
-            int v = list[a], b = a - 1;
-            while (b >= 0 && list[b] > v) { list[b + 1] = list[b]; --b; }
-            list[b + 1] = v;
-        }
-    }
-    int pmax = horizontal ? M : N;
+    if (lane == 0) cand_cnt[line] = 0;
+    const int pmax = horizontal ? M : N;
     double *tz = d.t_z, *Hm = d.H_moi, *FS = d.FS;
     const double *z = d.z, *ht = d.htaiz;
     int* kh = d.khouot;
+    unsigned char* dryflag = d.dryflag;
     auto at = [&](int k) -> int { return horizontal ? line * W + k : k * W + line; };
     /* open cell q from a source with level tzs and concentration fss (the
      * reference copies t_z and FS from the source, H_moi = htaiz + t_z) */
@@ -1997,35 +1993,114 @@ __global__ void k_wetting(const HdDev* __restrict__ dv, int horizontal)
             atomicOr((unsigned long long*)d.wetb + (size_t)r * ((M + 3 + 63) >> 6) + (cc >> 6), 1ull << (cc & 63));
         }
     };
-    /* visit positions in scan order: the recorded ones, and after a cell
-     * opens its right neighbour, that neighbour (the reference scan's
-     * cascade), which carries the source's t_z and FS.  Everything a visit
-     * reads goes out in one round of loads, before its stores. */
-    int li = 0, pos = 1;
+    if (n > kCand || n <= kWetFew) {
+        /* more candidates than recorded: the whole line; or so few that
+         * lane 0 visits them directly, one round of loads per visit */
+        if (lane != 0) return;
+        const bool full = n > kCand;
+        int c0 = 0, c1 = 0;
+        if (!full) {
+            c0 = d.cand[line * kCand];
+            c1 = n > 1 ? d.cand[line * kCand + 1] : c0;
+            if (c1 < c0) { const int t = c0; c0 = c1; c1 = t; }
+        }
+        int li = 0;
+        bool cascade = false;
+        double tzc = 0, fsc = 0;
+        for (int pos = full ? 2 : c0; pos <= pmax;) {
+            const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
+            const double zc = z[pp];
+            const double tzv = cascade ? tzc : tz[pp];
+            const double fsv = cascade ? fsc : FS[pp];
+            const int km = kh[pm], kn = kh[pn];
+            const double htm = ht[pm], htn = ht[pn];
+            cascade = false;
+            if (tzv > zc) {
+                if (km == 1) open(pos - 1, tzv, fsv, htm);
+                if (kn == 1) {
+                    open(pos + 1, tzv, fsv, htn);
+                    cascade = true;
+                    tzc = tzv;
+                    fsc = fsv;
+                }
+            }
+            if (full || cascade) {
+                ++pos;
+            } else { /* the next recorded position after pos */
+                while (li < n && (li == 0 ? c0 : c1) <= pos) ++li;
+                if (li >= n) break;
+                pos = li == 0 ? c0 : c1;
+            }
+        }
+        return;
+    }
+    __shared__ int s_pos[kCand];
+    __shared__ int s_kh[kCand][kWetWin];
+    __shared__ double s_ht[kCand][kWetWin], s_z[kCand][kWetLA + 1], s_tz[kCand], s_fs[kCand];
+    /* the recorded positions (distinct), sorted by rank */
+    const int c = lane < n ? d.cand[line * kCand + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += __shfl(c, k, kWave) < c;
+    if (lane < n) s_pos[rank] = c;
+    __syncthreads();
+    if (lane < n) { /* the window of one position, all loads at once */
+        const int p = s_pos[lane];
+        s_tz[lane] = tz[at(p)];
+        s_fs[lane] = FS[at(p)];
+#pragma unroll
+        for (int o = 0; o < kWetWin; ++o) {
+            const int q = p - 1 + o;
+            const bool in = q <= pmax + 1;
+            s_kh[lane][o] = in ? kh[at(q)] : 2;
+            s_ht[lane][o] = in ? ht[at(q)] : 0.0;
+        }
+#pragma unroll
+        for (int o = 0; o <= kWetLA; ++o) s_z[lane][o] = p + o <= pmax ? z[at(p + o)] : 0.0;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    /* the scan: recorded positions in order, cascades after right openings */
+    int li = 0, pos = 1, bw = 0;
+    int lastR = -8, prevR = -8; /* the last two cells opened from the left neighbour */
     bool cascade = false;
     double tzc = 0, fsc = 0;
     while (true) {
         if (cascade) {
             ++pos;
-        } else if (full) {
-            ++pos;
         } else {
-            while (li < cnt && list[li] <= pos) ++li;
-            if (li >= cnt) break;
-            pos = list[li];
+            while (li < n && s_pos[li] <= pos) ++li;
+            if (li >= n) break;
+            pos = s_pos[li];
+            bw = li;
         }
         if (pos < 2 || pos > pmax) break;
-        const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
-        const double zc = z[pp];
-        const double tzv = cascade ? tzc : tz[pp];
-        const double fsv = cascade ? fsc : FS[pp];
-        const int km = kh[pm], kn = kh[pn];
-        const double htm = ht[pm], htn = ht[pn];
+        const int o = pos - s_pos[bw];
+        double zc, htm, htn;
+        int km, kn;
+        if (o <= kWetLA) {
+            zc = s_z[bw][o];
+            km = s_kh[bw][o];
+            kn = s_kh[bw][o + 2];
+            htm = s_ht[bw][o];
+            htn = s_ht[bw][o + 2];
+        } else {
+            const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
+            zc = z[pp];
+            km = kh[pm];
+            kn = kh[pn];
+            htm = ht[pm];
+            htn = ht[pn];
+        }
+        if (pos - 1 == lastR || pos - 1 == prevR) km = 0;
+        const double tzv = cascade ? tzc : s_tz[bw];
+        const double fsv = cascade ? fsc : s_fs[bw];
         cascade = false;
         if (tzv > zc) {
             if (km == 1) open(pos - 1, tzv, fsv, htm);
             if (kn == 1) {
                 open(pos + 1, tzv, fsv, htn);
+                prevR = lastR;
+                lastR = pos + 1;
                 cascade = true;
                 tzc = tzv;
                 fsc = fsv;
@@ -2805,8 +2880,8 @@ void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t
 
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)
 {
-    int n = (horizontal ? b.N : b.M) + 1;
-    hipLaunchKernelGGL(k_wetting, dim3((n + 63) / 64, 1, b.n), dim3(64), 0, s, b.dv, horizontal);
+    int n = (horizontal ? b.N : b.M) + 1; /* lines 1..N (1..M), one wave each */
+    hipLaunchKernelGGL(k_wetting, dim3(n, 1, b.n), dim3(64), 0, s, b.dv, horizontal);
 }
 
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s)
