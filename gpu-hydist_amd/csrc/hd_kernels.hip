@@ -1950,11 +1950,11 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
  * opening q+1 makes q+1 the next visit, with q's values (the reference's
  * cascade).  Within a window every value is current except khouot at q-1
  * when an earlier visit opened it from the left, which is tracked; a
- * cascade past the window reads global memory.  A line with one or two
- * candidates is visited directly by lane 0, one with more candidates than
- * the list holds scanned in full. */
+ * cascade past the window reads global memory.  With up to kWetFew
+ * candidates lane 0 loads as it visits (one round of loads per visit); a
+ * line with more candidates than the list holds is scanned in full. */
 constexpr int kWetLA = 3;           /* cascade steps a window covers */
-constexpr int kWetFew = 2;          /* up to this many candidates lane 0 visits them directly */
+constexpr int kWetFew = 8;          /* up to this many candidates lane 0 loads as it visits */
 constexpr int kWetWin = kWetLA + 3; /* khouot / htaiz entries p-1 .. p+kWetLA+1 */
 
 __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal)
@@ -1993,21 +1993,11 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
             atomicOr((unsigned long long*)d.wetb + (size_t)r * ((M + 3 + 63) >> 6) + (cc >> 6), 1ull << (cc & 63));
         }
     };
-    if (n > kCand || n <= kWetFew) {
-        /* more candidates than recorded: the whole line; or so few that
-         * lane 0 visits them directly, one round of loads per visit */
+    if (n > kCand) { /* more candidates than recorded: the whole line, one visit per round of loads */
         if (lane != 0) return;
-        const bool full = n > kCand;
-        int c0 = 0, c1 = 0;
-        if (!full) {
-            c0 = d.cand[line * kCand];
-            c1 = n > 1 ? d.cand[line * kCand + 1] : c0;
-            if (c1 < c0) { const int t = c0; c0 = c1; c1 = t; }
-        }
-        int li = 0;
         bool cascade = false;
         double tzc = 0, fsc = 0;
-        for (int pos = full ? 2 : c0; pos <= pmax;) {
+        for (int pos = 2; pos <= pmax; ++pos) {
             const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
             const double zc = z[pp];
             const double tzv = cascade ? tzc : tz[pp];
@@ -2024,13 +2014,6 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
                     fsc = fsv;
                 }
             }
-            if (full || cascade) {
-                ++pos;
-            } else { /* the next recorded position after pos */
-                while (li < n && (li == 0 ? c0 : c1) <= pos) ++li;
-                if (li >= n) break;
-                pos = li == 0 ? c0 : c1;
-            }
         }
         return;
     }
@@ -2043,7 +2026,8 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
     for (int k = 0; k < n; ++k) rank += __shfl(c, k, kWave) < c;
     if (lane < n) s_pos[rank] = c;
     __syncthreads();
-    if (lane < n) { /* the window of one position, all loads at once */
+    const bool win = n > kWetFew; /* with few candidates lane 0 loads as it visits */
+    if (win && lane < n) { /* the window of one position, all loads at once */
         const int p = s_pos[lane];
         s_tz[lane] = tz[at(p)];
         s_fs[lane] = FS[at(p)];
@@ -2075,25 +2059,27 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
         }
         if (pos < 2 || pos > pmax) break;
         const int o = pos - s_pos[bw];
-        double zc, htm, htn;
+        double zc, htm, htn, tzv, fsv;
         int km, kn;
-        if (o <= kWetLA) {
+        if (win && o <= kWetLA) {
             zc = s_z[bw][o];
             km = s_kh[bw][o];
             kn = s_kh[bw][o + 2];
             htm = s_ht[bw][o];
             htn = s_ht[bw][o + 2];
-        } else {
+            tzv = cascade ? tzc : s_tz[bw];
+            fsv = cascade ? fsc : s_fs[bw];
+            if (pos - 1 == lastR || pos - 1 == prevR) km = 0;
+        } else { /* current memory (this lane's own stores included) */
             const int pp = at(pos), pm = at(pos - 1), pn = at(pos + 1);
             zc = z[pp];
             km = kh[pm];
             kn = kh[pn];
             htm = ht[pm];
             htn = ht[pn];
+            tzv = cascade ? tzc : tz[pp];
+            fsv = cascade ? fsc : FS[pp];
         }
-        if (pos - 1 == lastR || pos - 1 == prevR) km = 0;
-        const double tzv = cascade ? tzc : s_tz[bw];
-        const double fsv = cascade ? fsc : s_fs[bw];
         cascade = false;
         if (tzv > zc) {
             if (km == 1) open(pos - 1, tzv, fsv, htm);
