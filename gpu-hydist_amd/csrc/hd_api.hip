@@ -354,6 +354,7 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
         if ((rc = dalloc(c, (void**)&d.tiles, (size_t)(N + 3) * ncb * 4))) { hd_destroy(c); return rc; }
         if ((rc = dalloc(c, (void**)&d.ntiles, 4))) { hd_destroy(c); return rc; }
         if ((rc = dalloc(c, (void**)&d.wetb, (size_t)(N + 3) * ncb * 8))) { hd_destroy(c); return rc; }
+        if ((rc = dalloc(c, (void**)&d.kh0b, (size_t)(N + 3) * ncb * 8))) { hd_destroy(c); return rc; }
     }
     c->t_u0 = d.t_u;
     c->t_v0 = d.t_v;
@@ -619,9 +620,15 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
                 launch(x, "margins_v", [&] { hd_launch_margins(x.b(), 0, s); });
             }
         }
-        launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, ntiles, s); });
-        x.swap_v();
-        launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, s); });
+        if (x.solver == 0) {
+            /* smoothing of t_v + update_h_moi and the row wetting in one launch */
+            launch(x, "smooth_v_wetting_rows", [&] { hd_launch_smooth_wet(x.b(), 0, 1, ntiles, s); });
+            x.swap_v();
+        } else {
+            launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, ntiles, s); });
+            x.swap_v();
+            launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, s); });
+        }
     } else {
         int shift = (channel && o.kenhhepng == 1) ? 1 : 0;
         slo = 2 + shift;
@@ -650,9 +657,15 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
                 launch(x, "margins_u", [&] { hd_launch_margins(x.b(), 1, s); });
             }
         }
-        launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, ntiles, s); });
-        x.swap_u();
-        launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
+        if (x.solver == 0) {
+            /* smoothing of t_u + update_h_moi and the column wetting in one launch */
+            launch(x, "smooth_u_wetting_cols", [&] { hd_launch_smooth_wet(x.b(), 1, 0, ntiles, s); });
+            x.swap_u();
+        } else {
+            launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, ntiles, s); });
+            x.swap_u();
+            launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
+        }
     }
     if (x.solver == 0) {
         launch(x, "uvz_segments", [&] { hd_launch_uvz_segments(x.b(), o.kenhhepd, o.kenhhepng, ntiles, count_wet, s); });

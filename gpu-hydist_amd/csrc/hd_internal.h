@@ -38,7 +38,8 @@
  * the device time; tiles / ntiles the active 64-column row tiles; wetb the
  * wet bits of the segment finder, one 64-bit word per 64-cell tile row
  * ((N+3) x ceil((M+3)/64) words, bit b of word (i, w) for cell
- * (i, 64w + b): khouot == 0 and no drying flag); cand_cnt / cand the
+ * (i, 64w + b): khouot == 0 and no drying flag); kh0b the same layout for
+ * khouot == 0 before the wetting pass of the half-step; cand_cnt / cand the
  * wetting candidates (64 per line); dryflag one byte per cell. */
 #define HD_DEV_MEMBERS(G) \
     int N, M, W, SL, total_time, series_len; \
@@ -62,6 +63,7 @@
     G int* tiles; \
     G int* ntiles; \
     G unsigned long long* wetb; \
+    G unsigned long long* kh0b; \
     G int *cand_cnt, *cand; \
     G unsigned char* dryflag;
 
@@ -138,6 +140,7 @@ void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t
 void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
+void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, hipStream_t s);
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
 void hd_launch_vth_hesok(const HdBatch& b, int ntiles, hipStream_t s);
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);

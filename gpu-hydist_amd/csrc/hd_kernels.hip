@@ -1846,12 +1846,67 @@ __device__ __forceinline__ int margin_at(const HdDevG& d, int line, int x, bool 
     return 1;
 }
 
+/* The drying test of Reset_states (support_funcs.cu:114-127 / 173-185) and
+ * the wetting candidates at cell (i, j), as the drying pass (k_normalize with
+ * post) computes them, but without the H_moi write (update_h_moi,
+ * support_funcs.cu:60-73): the explicit cells of the same launch read the
+ * old H_moi, and the smoothing + wetting launch that follows writes it.
+ * dry_load does the loads, dry_store the drying flag, the wave's words of
+ * the wet bits (khouot == 0 and not drying) and of the pre-wetting mask bits
+ * kh0b (khouot == 0), and the candidate record; every lane of the wave that
+ * holds a grid cell calls dry_store. */
+struct DryMark {
+    int kh, kcm, kcp;
+    double htz, tz, zz;
+};
+
+__device__ __forceinline__ DryMark dry_load(const HdDevG& d, int i, int j, int horizontal)
+{
+    const int N = d.N, M = d.M, W = d.W, p = i * W + j;
+    const bool cin = horizontal ? (i >= 1 && i <= N && j >= 2 && j <= M) : (j >= 1 && j <= M && i >= 2 && i <= N);
+    const int step = horizontal ? 1 : W;
+    DryMark m;
+    m.kh = d.khouot[p];
+    m.htz = d.htaiz[p];
+    m.tz = d.t_z[p];
+    m.zz = d.z[p];
+    m.kcm = d.khouot[cin ? p - step : p];
+    m.kcp = d.khouot[cin ? p + step : p];
+    return m;
+}
+
+__device__ __forceinline__ void dry_store(const HdDevG& d, int i, int j, int horizontal, const DryMark& m)
+{
+    const int N = d.N, M = d.M, W = d.W, p = i * W + j;
+    unsigned char dry = 0;
+    if (i >= 1 && i <= N && j >= 1 && j <= M && m.kh == 0) {
+        const double hm = m.htz + m.tz;
+        const bool inrange = horizontal ? (j >= 2) : (i >= 2);
+        dry = inrange && (hm <= d.P.H_TINH);
+    }
+    d.dryflag[p] = dry;
+    const unsigned long long wb = __ballot(m.kh == 0 && !dry), kb = __ballot(m.kh == 0);
+    if ((threadIdx.x & 63) == 0) {
+        const size_t w = (size_t)i * ((M + 3 + 63) >> 6) + (j >> 6);
+        d.wetb[w] = wb;
+        d.kh0b[w] = kb;
+    }
+    const bool cin = horizontal ? (i >= 1 && i <= N && j >= 2 && j <= M) : (j >= 1 && j <= M && i >= 2 && i <= N);
+    if (cin && m.tz > m.zz && (m.kcm == 1 || m.kcp == 1)) {
+        const int ln = horizontal ? i : j, pos = horizontal ? j : i;
+        const int slot = atomicAdd(&d.cand_cnt[ln], 1);
+        if (slot < kCand) d.cand[ln * kCand + slot] = pos;
+    }
+}
+
 /* The UZ half-step's smoothing of t_u and solveV (UVZSolver_multithread.cu:
  * 1055-1071) in one pass over the active row tiles: the cells first ..
  * last-1 of every wet run of every row, recognised from the mask as in
  * k_explicit_rows_u (k and k+1 wet, and for k = M-1 also M-2), with the
  * smoothed t_u at the four faces around the cell recomputed from the
- * unsmoothed buffer, and the margins of the rows' runs (margin_at). */
+ * unsmoothed buffer, the margins of the rows' runs (margin_at), and the
+ * drying flags, wet bits and wetting candidates of the row reset
+ * (dry_load / dry_store). */
 __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled)
 {
     const HdDevG& d = dev_view(dv);
@@ -1872,6 +1927,7 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
         wl1 = kh[i * W + M - 1] == 0;
     }
     const bool in = rowok && j >= 2 && j <= M - 1 && wx && wn && (j <= M - 2 || wl2);
+    const DryMark dm = dry_load(d, i, j, 1);
     double vn = 0;
     if (in) {
         ExRaw r = v_solver_ld(d, i, j);
@@ -1883,6 +1939,7 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
     }
     d.tmp_u[p] = sm;
     if (in) d.t_v[p] = vn;
+    dry_store(d, i, j, 1, dm);
     if (rowok) {
         int mp[3];
         double mv[3];
@@ -1896,7 +1953,8 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
  * over the active row tiles: solveU reads the smoothed t_v at the four faces
  * around its cell, which this thread recomputes from the unsmoothed buffer
  * (the values the smoothing pass writes), so the two need no launch
- * boundary between them; the margins of the columns' runs (margin_at)
+ * boundary between them; the margins of the columns' runs (margin_at) and
+ * the drying flags, wet bits and wetting candidates of the column reset
  * too. */
 __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled)
 {
@@ -1918,6 +1976,7 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
         wl1 = kh[(N - 1) * W + j] == 0;
     }
     const bool in = colok && i >= 2 && i <= N - 1 && wx && wn && (i <= N - 2 || wl2);
+    const DryMark dm = dry_load(d, i, j, 0);
     double un = 0;
     if (in) {
         ExRaw r = u_solver_ld(d, i, j);
@@ -1929,6 +1988,7 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
     }
     d.tmp_v[p] = sm;
     if (in) d.t_u[p] = un;
+    dry_store(d, i, j, 0, dm);
     if (colok) {
         int mp[3];
         double mv[3];
@@ -1957,13 +2017,9 @@ constexpr int kWetLA = 3;           /* cascade steps a window covers */
 constexpr int kWetFew = 8;          /* up to this many candidates lane 0 loads as it visits */
 constexpr int kWetWin = kWetLA + 3; /* khouot / htaiz entries p-1 .. p+kWetLA+1 */
 
-__global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal)
+__device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, int line)
 {
-    const HdDevG& d = dev_view(dv);
-    /* the run tables are consumed for this half-step (solver and sediment
-     * sweeps); the segment finder at its end appends from zero */
-    if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
-    const int lane = threadIdx.x, line = blockIdx.x;
+    const int lane = threadIdx.x;
     const int N = d.N, M = d.M, W = d.W;
     if (line < 1 || line > (horizontal ? N : M)) return;
     int* cand_cnt = d.cand_cnt;
@@ -2093,6 +2149,75 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal)
+{
+    const HdDevG& d = dev_view(dv);
+    /* the run tables are consumed for this half-step (solver and sediment
+     * sweeps); the segment finder at its end appends from zero */
+    if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+    wetting_line(d, horizontal, blockIdx.x);
+}
+
+/* The second smoothing pass of a half-step (Normalize + update_buffer of the
+ * velocity the explicit pass produced, support_funcs.cu:682-729, into the
+ * partner buffer) with update_h_moi (support_funcs.cu:60-73), and the
+ * wetting pass of Reset_states, in one launch: workgroups [0, nlb) are the
+ * wetting lines (wetting_line), the others smooth one 64-cell tile row each
+ * (or every row of the grid when not tiled).  The smoothing weighs the
+ * neighbours across by the mask before the wetting (the kh0b bits the
+ * explicit pass wrote), and update_h_moi covers the cells wet before the
+ * wetting (which writes H_moi of the cells it opens itself), so neither half
+ * reads what the other writes. */
+__global__ void __launch_bounds__(64) k_smooth_wet(const HdDev* __restrict__ dv, int isU, int horizontal, int tiled,
+                                                    int nlb)
+{
+    const HdDevG& d = dev_view(dv);
+    if ((int)blockIdx.x < nlb) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
+        wetting_line(d, horizontal, blockIdx.x);
+        return;
+    }
+    const int t = (int)blockIdx.x - nlb;
+    const int ncb = (d.M + 3 + 63) >> 6;
+    int tile;
+    if (tiled) {
+        if (t >= d.ntiles[0]) return;
+        tile = d.tiles[t];
+    } else {
+        if (t >= (d.N + 3) * ncb) return;
+        tile = t;
+    }
+    const int i = tile / ncb, j = (tile % ncb) * 64 + (int)threadIdx.x;
+    const int N = d.N, M = d.M, W = d.W;
+    if (j >= M + 3) return;
+    const int p = i * W + j;
+    const double* val = isU ? d.t_u : d.t_v;
+    double* out = isU ? d.tmp_u : d.tmp_v;
+    const int close = isU ? W : 1;
+    const bool in = i >= 2 && i <= N && j >= 2 && j <= M;
+    /* the neighbours across: (i, j -+ 1) for u, (i -+ 1, j) for v */
+    const int fim = isU ? i : i - 1, fjm = isU ? j - 1 : j, fip = isU ? i : i + 1, fjp = isU ? j + 1 : j;
+    auto kh0 = [&](int qi, int qj) -> bool {
+        return (d.kh0b[(size_t)qi * ncb + (qj >> 6)] >> (qj & 63)) & 1;
+    };
+    const double r0 = val[p];
+    const double vcm = val[in ? p - close : p], vcp = val[in ? p + close : p];
+    const double vfm = val[in ? fim * W + fjm : p], vfp = val[in ? fip * W + fjp : p];
+    const bool kfm = in ? kh0(fim, fjm) : false, kfp = in ? kh0(fip, fjp) : false;
+    const bool wet0 = kh0(i, j);
+    const double htz = d.htaiz[p], tz = d.t_z[p];
+    double r = r0;
+    if (in && r != 0) {
+        int count = 2;
+        double nb = vcm + vcp;
+        if (kfm) { nb += vfm; count++; }
+        if (kfp) { nb += vfp; count++; }
+        r = r * d.P.heso + (1 - d.P.heso) * nb / count;
+    }
+    out[p] = r;
+    if (i >= 1 && i <= N && j >= 1 && j <= M && wet0) d.H_moi[p] = htz + tz;
 }
 
 /* update_uvz (support_funcs.cu:671-680) fused with the drying writes of
@@ -2862,6 +2987,14 @@ void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStre
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
 {
     hipLaunchKernelGGL(k_explicit_rows_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
+}
+
+void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, hipStream_t s)
+{
+    const int nlb = (horizontal ? b.N : b.M) + 1; /* lines 1..N (1..M), one wave each */
+    const int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ((b.M + 3 + 63) / 64);
+    hipLaunchKernelGGL(k_smooth_wet, dim3(nlb + tiles, 1, b.n), dim3(64), 0, s, b.dv, isU, horizontal,
+                       ntiles > 0 ? 1 : 0, nlb);
 }
 
 void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)
