@@ -45,9 +45,10 @@ KERNEL_BYTES = {
     "explicit_margins_u": 9 * 8 + 8,
     "explicit_rows_u": 9 * 8 + 8,
     # smoothing of t_v with explicit u in one pass: t_v, khouot read, the
-    # partner written, plus explicit u's 80 B
-    "smooth_v_explicit_u": 8 + 4 + 8 + 9 * 8 + 8,
-    "smooth_u_explicit_v": 8 + 4 + 8 + 9 * 8 + 8,
+    # partner written, plus explicit u's 80 B, plus the drying marks (htaiz,
+    # t_z, z read, the drying flag written)
+    "smooth_v_explicit_u": 8 + 4 + 8 + 9 * 8 + 8 + 3 * 8 + 1,
+    "smooth_u_explicit_v": 8 + 4 + 8 + 9 * 8 + 8 + 3 * 8 + 1,
     # state copy + drying + face depths: t_z, t_u, t_v, H_moi, h read;
     # z, u, v, Htdu, Htdv written
     "uvz_dry_htd": 5 * 8 + 5 * 8,
@@ -57,6 +58,10 @@ KERNEL_BYTES = {
     "smooth_v": 8 + 4 + 8,
     "smooth_v_hmoi": 8 + 4 + 8 + 2 * 8 + 8,
     "smooth_u_hmoi": 8 + 4 + 8 + 2 * 8 + 8,
+    # the second smoothing (velocity read, partner written) with update_h_moi
+    # (htaiz, t_z read, H_moi written) beside the wetting lines
+    "smooth_v_wetting_rows": 8 + 8 + 2 * 8 + 8,
+    "smooth_u_wetting_cols": 8 + 8 + 2 * 8 + 8,
     # wet-run tables: khouot read
     "segments_fs_edges": 4,
     # the two in one launch (state copy + tables)
