@@ -2001,7 +2001,8 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
 
 /* Wetting pass of Reset_states_horizontal (support_funcs.cu:92-112) /
  * _vertical (:152-171), one wave per line.  The serial scan can only act at
- * positions next to a khouot == 1 cell; those were recorded by k_normalize.
+ * positions next to a khouot == 1 cell; those were recorded by the drying
+ * marks (dry_store, or k_normalize with post in the cross-check forms).
  * The wave sorts them (ranks by comparison), every lane loads the window of
  * one recorded position p at once (khouot and htaiz at p-1 .. p+kWetLA+1, z
  * at p .. p+kWetLA, t_z and FS at p), and lane 0 replays the scan over the
