@@ -20,7 +20,8 @@ constexpr int kSpecWave = 64;
  * the linear recurrence v_i = M_i v_{i-1},
  *   M_i = [[0, -C_i, 0], [A_i, B_i, 0], [0, D_i, -A_i]],
  * so each lane composes the matrices of its own rows, an inclusive scan over
- * the wave composes the prefixes, and every lane obtains an estimate, good to
+ * the wave (DPP row shifts and broadcasts) composes the prefixes, and every
+ * lane obtains an estimate, good to
  * a few ulps, of the state after each earlier lane's rows.  The recurrence
  * contracts on these systems (each continuity / momentum pair reduces to a
  * diagonally dominant system in z), so two rounded trajectories that start a
@@ -89,14 +90,40 @@ __device__ __forceinline__ void mx_norm(Mx& X)
     }
 }
 
-__device__ __forceinline__ Mx mx_shfl_up(const Mx& X, int dd)
+/* A 64-bit value moved across lanes by a DPP mode (row shifts within rows
+ * of 16 lanes, row broadcasts to later rows): a lane the mode gives no source,
+ * or whose row ROWMASK leaves out, gets `old`. */
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double old, double src)
+{
+    const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+/* One step of the wave's inclusive prefix scan of row matrices: X := X Y
+ * with Y the matrix of the source lane, the identity where there is none. */
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void mx_scan_step(Mx& X)
 {
     Mx Y;
-    Y.a = __shfl_up(X.a, dd, kSpecWave); Y.b = __shfl_up(X.b, dd, kSpecWave);
-    Y.c = __shfl_up(X.c, dd, kSpecWave); Y.d = __shfl_up(X.d, dd, kSpecWave);
-    Y.e = __shfl_up(X.e, dd, kSpecWave); Y.f = __shfl_up(X.f, dd, kSpecWave);
-    Y.g = __shfl_up(X.g, dd, kSpecWave);
-    return Y;
+    Y.a = dpp_f64<CTRL, ROWMASK>(1.0, X.a); Y.b = dpp_f64<CTRL, ROWMASK>(0.0, X.b);
+    Y.c = dpp_f64<CTRL, ROWMASK>(0.0, X.c); Y.d = dpp_f64<CTRL, ROWMASK>(1.0, X.d);
+    Y.e = dpp_f64<CTRL, ROWMASK>(0.0, X.e); Y.f = dpp_f64<CTRL, ROWMASK>(0.0, X.f);
+    Y.g = dpp_f64<CTRL, ROWMASK>(1.0, X.g);
+    X = mx_mul(X, Y);
+    mx_norm(X);
+}
+
+/* The same for the affine maps x -> al x + be of back substitution: the
+ * source lane's map is applied first. */
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void aff_scan_step(double& al, double& be)
+{
+    const double a2 = dpp_f64<CTRL, ROWMASK>(1.0, al), b2 = dpp_f64<CTRL, ROWMASK>(0.0, be);
+    be = fma(al, b2, be);
+    al = al * a2;
 }
 
 __device__ __forceinline__ bool same_bits(double x, double y)
@@ -157,13 +184,16 @@ __device__ __forceinline__ void spec_forward(const double* sA, const double* sB,
         if (((i - sk) & 3) == 3) mx_norm(X);
     }
     mx_norm(X);
-    for (int dd = 1; dd < kSpecWave; dd <<= 1) {
-        Mx Y = mx_shfl_up(X, dd);
-        if (lane >= dd) {
-            X = mx_mul(X, Y);
-            mx_norm(X);
-        }
-    }
+    /* inclusive scan over the wave with DPP moves: shifts by 1, 2, 4, 8
+     * within rows of 16 lanes, then the end of row r to row r + 1 (rows 1
+     * and 3), then the end of row 1 to rows 2 and 3; the estimates need not
+     * match any particular rounding */
+    mx_scan_step<0x111, 0xF>(X);
+    mx_scan_step<0x112, 0xF>(X);
+    mx_scan_step<0x114, 0xF>(X);
+    mx_scan_step<0x118, 0xF>(X);
+    mx_scan_step<0x142, 0xA>(X);
+    mx_scan_step<0x143, 0xC>(X);
     double ea, eb;
     {
         double p = fma(X.a, ap, X.b), q = fma(X.c, ap, X.d), r = fma(X.e, ap, fma(X.g, bp, X.f));
@@ -247,13 +277,13 @@ __device__ __forceinline__ double spec_backward(const double* sAp, const double*
             be = fma(sAp[i], be, sBp[i]);
             al = sAp[i] * al;
         }
-        for (int dd = 1; dd < kSpecWave; dd <<= 1) {
-            double a2 = __shfl_up(al, dd, kSpecWave), b2 = __shfl_up(be, dd, kSpecWave);
-            if (lane >= dd) {
-                be = fma(al, b2, be);
-                al = al * a2;
-            }
-        }
+        /* inclusive scan of the affine maps, DPP moves as in spec_forward */
+        aff_scan_step<0x111, 0xF>(al, be);
+        aff_scan_step<0x112, 0xF>(al, be);
+        aff_scan_step<0x114, 0xF>(al, be);
+        aff_scan_step<0x118, 0xF>(al, be);
+        aff_scan_step<0x142, 0xA>(al, be);
+        aff_scan_step<0x143, 0xC>(al, be);
         double est = fma(al, x_in, be);
         est = __shfl(est, j < 0 ? 0 : j, kSpecWave);
         if (j >= 0) { cx = est; i0 = hi - (j + 1) * L; }
