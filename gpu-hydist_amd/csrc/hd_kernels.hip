@@ -1792,17 +1792,46 @@ __device__ __forceinline__ double explicit_u_at(const HdDevG& d, int x, int j)
  * margins first-1 and last, both 0 except at an open grid edge: first == 2
  * with the depth at first-1 not at the wall value gives 2 E(2) - E(3),
  * last == L likewise 2 E(L-1) - E(L-2), E being the line after solveU /
- * solveV.  E(L-1), E(L-2) are explicit cells of the run, computed here;
- * E(2), E(3) are too unless the run is [2, 2] or [2, 3], where the value
- * before the margins enters.  Then the run's margin last (and, after
- * [2, 2], the margin 3 of a run starting at 4) is written by position 1's
- * thread after it read them, in the reference's order, and not by their own
- * threads.  Returns the number of margin values this thread writes (0..3),
- * at positions pos[k] with values val[k]; only positions 1..3 and L load
- * anything. */
+ * solveV.  E(L-1), E(L-2) are explicit cells of the run, E(2), E(3) too
+ * unless the run is [2, 2] or [2, 3], where the value before the margins
+ * enters; the threads of positions 1 and L evaluate them (edge_vals, `ev`).
+ * In those short cases the run's margin last (and, after [2, 2], the margin
+ * 3 of a run starting at 4) is written by position 1's thread after it read
+ * them, in the reference's order, and not by their own threads.  Returns the
+ * number of margin values this thread writes (0..3), at positions pos[k]
+ * with values val[k]; only positions 1..3 and L load anything. */
+struct EdgeVals {
+    double ea, eb, o2, o3; /* E(2), E(3), old 2, old 3 at position 1; E(L-1), E(L-2) at L */
+};
+
+/* The values an open-edge margin may use, evaluated with the pass's other
+ * loads by the threads of positions 1 and L, which have no explicit cell of
+ * their own.  The old values of positions 2 and 3 are used only where no
+ * other thread writes those positions. */
+template <bool ROWS>
+__device__ __forceinline__ EdgeVals edge_vals(const HdDevG& d, int line, int x)
+{
+    EdgeVals e{0.0, 0.0, 0.0, 0.0};
+    const int W = d.W, L = ROWS ? d.M : d.N;
+    if (line < 2 || line > (ROWS ? d.N : d.M)) return e;
+    auto at = [&](int k) { return ROWS ? line * W + k : k * W + line; };
+    auto ex_at = [&](int k) -> double { return ROWS ? explicit_v_at(d, line, k) : explicit_u_at(d, k, line); };
+    if (x == 1) {
+        const double* t = ROWS ? d.t_v : d.t_u;
+        e.o2 = t[at(2)];
+        e.o3 = t[at(3)];
+        e.ea = ex_at(2);
+        e.eb = ex_at(3);
+    } else if (x == L) {
+        e.ea = ex_at(L - 1);
+        e.eb = ex_at(L - 2);
+    }
+    return e;
+}
+
 template <bool ROWS>
 __device__ __forceinline__ int margin_at(const HdDevG& d, int line, int x, bool wx, bool wn, bool wl2, bool wl1,
-                                         int pos[3], double val[3])
+                                         const EdgeVals& ev, int pos[3], double val[3])
 {
     const int N = d.N, M = d.M, W = d.W;
     const int L = ROWS ? M : N;
@@ -1815,16 +1844,14 @@ __device__ __forceinline__ int margin_at(const HdDevG& d, int line, int x, bool 
     if (x > 3 && x < L) return 1; /* first > 4, or last < L: a wall margin */
     const int* kh = d.khouot;
     const double* h = d.h;
-    const double* t = ROWS ? d.t_v : d.t_u;
     auto at = [&](int k) { return ROWS ? line * W + k : k * W + line; };
     auto wet = [&](int k) -> bool { return k >= 2 && k <= L && kh[at(k)] == 0; };
-    auto ex_at = [&](int k) -> double { return ROWS ? explicit_v_at(d, line, k) : explicit_u_at(d, k, line); };
     const double eps = 1e-16; /* UVZSolver_multithread.cu:12 */
     const double ND = d.P.NANGDAY;
     if (x == L) {
         const bool wall2 = ROWS ? fabs((h[line * W + M] + h[(line - 1) * W + M]) * 0.5 - ND) < eps
                                 : fabs((h[N * W + line] + h[N * W + line - 1]) * 0.5 - ND) < eps;
-        if (!wall2) val[0] = 2 * ex_at(L - 1) - ex_at(L - 2);
+        if (!wall2) val[0] = 2 * ev.ea - ev.eb;
         return 1;
     }
     /* x <= 3: the run from 2, if any -- does it end at 2 or 3, does its
@@ -1835,14 +1862,14 @@ __device__ __forceinline__ int margin_at(const HdDevG& d, int line, int x, bool 
     const bool end2 = w2 && !w3, end3 = w2 && w3 && !w4;
     if (w2 && !wall1 && (end2 || end3)) {
         if (x != 1) return 0;
-        const double e2 = end2 ? t[at(2)] : ex_at(2);
-        val[0] = 2 * e2 - t[at(3)];
+        const double e2 = end2 ? ev.o2 : ev.ea;
+        val[0] = 2 * e2 - ev.o3;
         pos[1] = end2 ? 2 : 3;
         val[1] = 0.0;
         if (end2 && w4 && 4 <= L - 2) { pos[2] = 3; val[2] = 0.0; return 3; }
         return 2;
     }
-    if (x == 1 && !wall1) val[0] = 2 * ex_at(2) - ex_at(3);
+    if (x == 1 && !wall1) val[0] = 2 * ev.ea - ev.eb;
     return 1;
 }
 
@@ -1937,13 +1964,14 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
         r.t3 = smooth_u_at(d, i, j + 1);
         vn = v_solver_ev(d.P, r);
     }
+    const EdgeVals ev = in ? EdgeVals{0.0, 0.0, 0.0, 0.0} : edge_vals<true>(d, i, j);
     d.tmp_u[p] = sm;
     if (in) d.t_v[p] = vn;
     dry_store(d, i, j, 1, dm);
     if (rowok) {
         int mp[3];
         double mv[3];
-        const int nm = margin_at<true>(d, i, j, wx, wn, wl2, wl1, mp, mv);
+        const int nm = margin_at<true>(d, i, j, wx, wn, wl2, wl1, ev, mp, mv);
         for (int k = 0; k < nm; ++k) d.t_v[i * W + mp[k]] = mv[k];
     }
 }
@@ -1986,13 +2014,14 @@ __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __rest
         r.t3 = smooth_v_at(d, i + 1, j);
         un = u_solver_ev(d.P, r);
     }
+    const EdgeVals ev = in ? EdgeVals{0.0, 0.0, 0.0, 0.0} : edge_vals<false>(d, j, i);
     d.tmp_v[p] = sm;
     if (in) d.t_u[p] = un;
     dry_store(d, i, j, 0, dm);
     if (colok) {
         int mp[3];
         double mv[3];
-        const int nm = margin_at<false>(d, j, i, wx, wn, wl2, wl1, mp, mv);
+        const int nm = margin_at<false>(d, j, i, wx, wn, wl2, wl1, ev, mp, mv);
         for (int k = 0; k < nm; ++k) d.t_u[mp[k] * W + j] = mv[k];
     }
 }
