@@ -111,6 +111,9 @@ struct hd_ctx {
      * kernel + one lane per line ("rows").  All four give identical bits. */
     int solver = 0;
     bool solver_v1 = false;     /* two-kernel sediment form as well (HYDIST_SOLVER=rows) */
+    /* wetting form (HYDIST_WETTING): 0 default, 1 candidate windows for every
+     * line ("window"), 2 whole-line scans ("scan"); identical bits */
+    int wet_mode = 0;
     int run_blocks_u = 0, run_blocks_v = 0;
     double* bed_snap = nullptr; /* dH snapshot for hd_bed_extrapolate (allocated on first use) */
     double* z_snap = nullptr;   /* z snapshot for hd_z_change (allocated on first use), + 3 result slots */
@@ -134,6 +137,7 @@ struct Exec {
     int state;
     int solver;
     bool solver_v1;
+    int wet_mode;
     int run_blocks_u, run_blocks_v;
     int bw;
     Prof* prof;
@@ -166,7 +170,7 @@ static void set_knobs(hd_ctx* c, int rb_mult, int bw)
 
 static Exec exec_of(hd_ctx* c)
 {
-    return Exec{c->stream, c->dv, 1, c->N, c->M, c->state(), c->solver, c->solver_v1, c->run_blocks_u,
+    return Exec{c->stream, c->dv, 1, c->N, c->M, c->state(), c->solver, c->solver_v1, c->wet_mode, c->run_blocks_u,
                 c->run_blocks_v, c->bw, &c->prof, c};
 }
 
@@ -283,6 +287,8 @@ int hd_create(int device, const hd_params* p, int N, int M, int series_len, int 
         const char* sv = getenv("HYDIST_SOLVER");
         c->solver_v1 = sv && std::string(sv) == "rows";
         c->solver = c->solver_v1 ? 2 : (sv && std::string(sv) == "line") ? 1 : (sv && std::string(sv) == "run") ? 3 : 0;
+        const char* wv = getenv("HYDIST_WETTING");
+        c->wet_mode = (wv && std::string(wv) == "window") ? 1 : (wv && std::string(wv) == "scan") ? 2 : 0;
     }
     int rc;
     if (hipSetDevice(device) != hipSuccess) { delete c; return fail(nullptr, HD_EHIP, "hipSetDevice failed"); }
@@ -622,12 +628,12 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         }
         if (x.solver == 0) {
             /* smoothing of t_v + update_h_moi and the row wetting in one launch */
-            launch(x, "smooth_v_wetting_rows", [&] { hd_launch_smooth_wet(x.b(), 0, 1, ntiles, s); });
+            launch(x, "smooth_v_wetting_rows", [&] { hd_launch_smooth_wet(x.b(), 0, 1, ntiles, x.wet_mode, s); });
             x.swap_v();
         } else {
             launch(x, "smooth_v_hmoi", [&] { hd_launch_normalize_post(x.b(), 0, 1, 1, x.bw, ntiles, s); });
             x.swap_v();
-            launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, s); });
+            launch(x, "wetting_rows", [&] { hd_launch_wetting(x.b(), 1, x.wet_mode, s); });
         }
     } else {
         int shift = (channel && o.kenhhepng == 1) ? 1 : 0;
@@ -659,12 +665,12 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         }
         if (x.solver == 0) {
             /* smoothing of t_u + update_h_moi and the column wetting in one launch */
-            launch(x, "smooth_u_wetting_cols", [&] { hd_launch_smooth_wet(x.b(), 1, 0, ntiles, s); });
+            launch(x, "smooth_u_wetting_cols", [&] { hd_launch_smooth_wet(x.b(), 1, 0, ntiles, x.wet_mode, s); });
             x.swap_u();
         } else {
             launch(x, "smooth_u_hmoi", [&] { hd_launch_normalize_post(x.b(), 1, 1, 0, x.bw, ntiles, s); });
             x.swap_u();
-            launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, s); });
+            launch(x, "wetting_cols", [&] { hd_launch_wetting(x.b(), 0, x.wet_mode, s); });
         }
     }
     if (x.solver == 0) {
@@ -1317,7 +1323,7 @@ int hd_ensemble_run(hd_ensemble* e, double t, int nsteps, const hd_step_opts* o,
         if (hipStreamSynchronize(c->stream) != hipSuccess) return efail(e, HD_EHIP, "member sync");
     }
     int count = counted ? 1 : 0;
-    Exec x{e->stream, e->dv, (int)e->m.size(), e->N, e->M, 0, c0->solver, c0->solver_v1, e->run_blocks_u,
+    Exec x{e->stream, e->dv, (int)e->m.size(), e->N, e->M, 0, c0->solver, c0->solver_v1, c0->wet_mode, e->run_blocks_u,
            e->run_blocks_v, c0->bw, &e->prof, nullptr};
     set_time(x, t);
     std::string why;

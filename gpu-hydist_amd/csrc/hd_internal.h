@@ -139,8 +139,9 @@ void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
 void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s);
-void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s);
-void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, hipStream_t s);
+/* mode: 0 the default wetting, 1 candidate windows for every line, 2 whole-line scans (cross-checks) */
+void hd_launch_wetting(const HdBatch& b, int horizontal, int mode, hipStream_t s);
+void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, int mode, hipStream_t s);
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s);
 void hd_launch_vth_hesok(const HdBatch& b, int ntiles, hipStream_t s);
 void hd_launch_fs_coef(const HdBatch& b, int which, double s_start, int ketdinh, hipStream_t s);

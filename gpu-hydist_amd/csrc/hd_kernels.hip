@@ -2047,7 +2047,7 @@ constexpr int kWetLA = 3;           /* cascade steps a window covers */
 constexpr int kWetFew = 8;          /* up to this many candidates lane 0 loads as it visits */
 constexpr int kWetWin = kWetLA + 3; /* khouot / htaiz entries p-1 .. p+kWetLA+1 */
 
-__device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, int line)
+__device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, int line, int mode)
 {
     const int lane = threadIdx.x;
     const int N = d.N, M = d.M, W = d.W;
@@ -2079,7 +2079,9 @@ __device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, in
             atomicOr((unsigned long long*)d.wetb + (size_t)r * ((M + 3 + 63) >> 6) + (cc >> 6), 1ull << (cc & 63));
         }
     };
-    if (n > kCand) { /* more candidates than recorded: the whole line, one visit per round of loads */
+    /* mode (a cross-check of the three ways, same bits): 0 as described,
+     * 1 windows for every line, 2 the whole-line scan for every line */
+    if (n > kCand || mode == 2) { /* more candidates than recorded: the whole line, one visit per round of loads */
         if (lane != 0) return;
         bool cascade = false;
         double tzc = 0, fsc = 0;
@@ -2112,7 +2114,7 @@ __device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, in
     for (int k = 0; k < n; ++k) rank += __shfl(c, k, kWave) < c;
     if (lane < n) s_pos[rank] = c;
     __syncthreads();
-    const bool win = n > kWetFew; /* with few candidates lane 0 loads as it visits */
+    const bool win = n > (mode == 1 ? 0 : kWetFew); /* with few candidates lane 0 loads as it visits */
     if (win && lane < n) { /* the window of one position, all loads at once */
         const int p = s_pos[lane];
         s_tz[lane] = tz[at(p)];
@@ -2181,13 +2183,13 @@ __device__ __forceinline__ void wetting_line(const HdDevG& d, int horizontal, in
     }
 }
 
-__global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal)
+__global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, int horizontal, int mode)
 {
     const HdDevG& d = dev_view(dv);
     /* the run tables are consumed for this half-step (solver and sediment
      * sweeps); the segment finder at its end appends from zero */
     if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
-    wetting_line(d, horizontal, blockIdx.x);
+    wetting_line(d, horizontal, blockIdx.x, mode);
 }
 
 /* The second smoothing pass of a half-step (Normalize + update_buffer of the
@@ -2201,12 +2203,12 @@ __global__ void __launch_bounds__(64) k_wetting(const HdDev* __restrict__ dv, in
  * wetting (which writes H_moi of the cells it opens itself), so neither half
  * reads what the other writes. */
 __global__ void __launch_bounds__(64) k_smooth_wet(const HdDev* __restrict__ dv, int isU, int horizontal, int tiled,
-                                                    int nlb)
+                                                    int nlb, int mode)
 {
     const HdDevG& d = dev_view(dv);
     if ((int)blockIdx.x < nlb) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { d.run_cnt[0] = 0; d.run_cnt[1] = 0; }
-        wetting_line(d, horizontal, blockIdx.x);
+        wetting_line(d, horizontal, blockIdx.x, mode);
         return;
     }
     const int t = (int)blockIdx.x - nlb;
@@ -3019,18 +3021,18 @@ void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t
     hipLaunchKernelGGL(k_explicit_rows_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
 }
 
-void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, hipStream_t s)
+void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, int mode, hipStream_t s)
 {
     const int nlb = (horizontal ? b.N : b.M) + 1; /* lines 1..N (1..M), one wave each */
     const int tiles = ntiles > 0 ? ntiles : (b.N + 3) * ((b.M + 3 + 63) / 64);
     hipLaunchKernelGGL(k_smooth_wet, dim3(nlb + tiles, 1, b.n), dim3(64), 0, s, b.dv, isU, horizontal,
-                       ntiles > 0 ? 1 : 0, nlb);
+                       ntiles > 0 ? 1 : 0, nlb, mode);
 }
 
-void hd_launch_wetting(const HdBatch& b, int horizontal, hipStream_t s)
+void hd_launch_wetting(const HdBatch& b, int horizontal, int mode, hipStream_t s)
 {
     int n = (horizontal ? b.N : b.M) + 1; /* lines 1..N (1..M), one wave each */
-    hipLaunchKernelGGL(k_wetting, dim3(n, 1, b.n), dim3(64), 0, s, b.dv, horizontal);
+    hipLaunchKernelGGL(k_wetting, dim3(n, 1, b.n), dim3(64), 0, s, b.dv, horizontal, mode);
 }
 
 void hd_launch_uvz_dry(const HdBatch& b, int kd, int kn, int bw, int ntiles, hipStream_t s)

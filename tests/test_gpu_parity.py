@@ -306,6 +306,27 @@ def test_solver_forms_identical(form, make, steps, monkeypatch):
     assert_same_state(ref, alt, f"speculative vs {form}")
 
 
+@pytest.mark.parametrize("form,make,steps", [("window", small_river, 40), ("scan", small_river, 40),
+                                             ("window", full_river, 40), ("scan", full_river, 20),
+                                             ("window", small_river_sediment, 30)])
+def test_wetting_forms_identical(form, make, steps, monkeypatch):
+    """The wetting pass's three ways -- candidates visited with their own
+    loads (up to 8 per line), replayed from prefetched windows, or the whole
+    line scanned -- give identical bits (HYDIST_WETTING forces one way for
+    every line)."""
+    case = make()
+    ref = Solver(case)
+    monkeypatch.setenv("HYDIST_WETTING", form)
+    alt = Solver(case)
+    monkeypatch.delenv("HYDIST_WETTING")
+    ref.init()
+    alt.init()
+    op = case.opts()
+    ref.run(0.0, steps, op)
+    alt.run(0.0, steps, op)
+    assert_same_state(ref, alt, f"default wetting vs {form}")
+
+
 def test_clone_copies_state_and_steps_identically():
     case = small_river()
     g = Solver(case)
