@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) k_fs_coef(const HdDev* __restrict__ dv, d
     __shared__ int s_moc[64];
     __shared__ int s_dau[64][24];
     __shared__ int s_cuoi[64][24];
-    int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    int N = d.N, M = d.M, SL = d.SL;
     int tx = threadIdx.x, ty = threadIdx.y;
     int j0 = blockIdx.x * 64, i0 = blockIdx.y * 16;
     int tid = ty * 64 + tx;
@@ -220,11 +220,8 @@ __global__ void __launch_bounds__(256) k_fs_coef(const HdDev* __restrict__ dv, d
     __syncthreads();
     int j = j0 + tx;
     if (j > M) return;
-    const hd_params& P = d.P;
     const double t = tdev[0];
     const int tw = 2 * (N > M ? N : M) + 4;
-    const double *FS = d.FS, *Hm = d.H_moi, *t_u = d.t_u, *t_v = d.t_v, *VTH = d.VTH, *Htdu = d.Htdu, *Htdv = d.Htdv,
-                 *Fw = d.Fw, *Kx = d.Kx, *Ky = d.Ky;
     for (int r = ty; r < 16; r += 4) {
         int i = i0 + r;
         if (i > N) break;
