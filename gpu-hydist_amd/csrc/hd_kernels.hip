@@ -1095,7 +1095,7 @@ __device__ __forceinline__ Pre shfl_pre(const Pre& a, int src)
  * together, so a short run costs one memory round trip after its table
  * entry. */
 template <bool isU>
-__global__ void __launch_bounds__(64, 3) k_adi_spec(const HdDev* __restrict__ dv, int shift)
+__global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv, int shift)
 {
     const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
