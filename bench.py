@@ -268,6 +268,10 @@ def main():
                "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
                          f"(FP64 restatement, OpenMP over lines, {used} threads of {os.cpu_count()} "
                          f"logical CPUs, {cpu_model()})"}
+        # the same checker on one thread, over half the time budget
+        rate1, n1, dt1, _ = cpu_baseline(case, a.cpu_seconds / 2, 1)
+        cpu["single_thread"] = {"value": rate1, "cores": 1,
+                                "sample": f"first {n1} steps of {case.name} from t=0 ({dt1:.1f} s)"}
 
     ens = None
     if not a.no_ensemble:
