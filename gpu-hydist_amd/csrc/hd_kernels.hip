@@ -2619,10 +2619,12 @@ __device__ __forceinline__ unsigned long long transpose64(unsigned long long a, 
  * over the waves.  Rows beyond 1024 are done in further passes with the
  * counts and the last bit carried.  Every thread returns its column's
  * number of entries (min(runs, SL)); entries go to dauj / cuoij and to
- * sf / sl [column * 24 + k]. */
+ * sf / sl [column * 24 + k], and s_cbase[column] receives the column's
+ * first slot in the packed run table (reserved in the last pass).  Ends
+ * with a barrier. */
 constexpr int kColKW = 4; /* words (64 rows each) per wave and pass */
 
-__device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, int* s_l)
+__device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, int* s_l, int* s_cbase)
 {
     const int N = d.N, M = d.M, SL = d.SL, L = N;
     const int ncb = (M + 3 + 63) >> 6;
@@ -2635,9 +2637,10 @@ __device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, 
     };
     __shared__ unsigned s_first[4][kWave], s_last[4][kWave], s_pb[kWave];
     __shared__ int s_cs[4][kWave], s_ce[4][kWave], s_ts[kWave], s_te[kWave];
-    const unsigned wl2 = wetx(L - 2), wl1 = wetx(L - 1);
     if (c == 0) { s_ts[lane] = 0; s_te[lane] = 0; s_pb[lane] = 0; }
     __syncthreads();
+    /* after the barrier, so that these loads travel with the first pass's */
+    const unsigned wl2 = wetx(L - 2), wl1 = wetx(L - 1);
     constexpr int kPass = 4 * kColKW * kWave;
     for (int x0 = 0; x0 <= L; x0 += kPass) {
         unsigned long long t[kColKW], st[kColKW], en[kColKW];
@@ -2672,6 +2675,23 @@ __device__ __forceinline__ int col_word_scan(const HdDevG& d, int cb, int* s_f, 
         s_cs[c][lane] = cs;
         s_ce[c][lane] = ce;
         __syncthreads();
+        if (c == 0 && x0 + kPass > L) {
+            /* last pass: the columns' entry counts are known; reserve their
+             * slots of the packed run table (column order, one atomic) while
+             * the entries are written */
+            int ns = s_ts[lane];
+            for (int w = 0; w < 4; ++w) ns += s_cs[w][lane];
+            const int nn = colok ? (ns < SL ? ns : SL) : 0;
+            int p = nn;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int a = __shfl_up(p, o, kWave);
+                if (lane >= o) p += a;
+            }
+            const int tot = __shfl(p, kWave - 1, kWave);
+            int base = 0;
+            if (lane == 0 && tot) base = atomicAdd(d.run_cnt + 0, tot);
+            s_cbase[lane] = __shfl(base, 0, kWave) + p - nn;
+        }
         int ks = s_ts[lane], ke = s_te[lane];
         for (int w = 0; w < c; ++w) { ks += s_cs[w][lane]; ke += s_ce[w][lane]; }
 #pragma unroll
@@ -2730,19 +2750,7 @@ __device__ __forceinline__ void find_limits_block(const HdDevG& d, int bid, int 
         __shared__ int s_cf[kWave * 24], s_cl[kWave * 24], s_cbase[kWave];
         const int cb = bid - nrowb;
         if (64 * cb > M) return; /* no column of this context (a smaller member of a batch) */
-        const int n = col_word_scan(d, cb, s_cf, s_cl);
-        if (wv == 0) { /* slots of the block's columns in column order, one atomic */
-            int p = n;
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int a = __shfl_up(p, o, kWave);
-                if (lane >= o) p += a;
-            }
-            const int tot = __shfl(p, kWave - 1, kWave);
-            int base = 0;
-            if (lane == 0 && tot) base = atomicAdd(d.run_cnt + 0, tot);
-            s_cbase[lane] = __shfl(base, 0, kWave) + p - n;
-        }
-        __syncthreads();
+        const int n = col_word_scan(d, cb, s_cf, s_cl, s_cbase); /* ends with a barrier */
         const int cap = (M + 3) * SL;
         const int j = 64 * cb + lane;
         for (int k = wv; k < n; k += 4) {
