@@ -517,7 +517,9 @@ __global__ void __launch_bounds__(64) k_fs_run(const HdDev* __restrict__ dv, dou
 }
 
 /* k_fs_run with the speculative parallel Thomas of hd_thomas.h: rows of a
- * run are assembled in chunks of 512 (eight per lane), the forward sweep and
+ * run are assembled in chunks of 256 (four per lane; 8 KB of LDS per
+ * workgroup, so that about four workgroups per SIMD fit where a batch has
+ * many more runs than the chip has waves), the forward sweep and
  * the back substitution run on all 64 lanes with exact verification, so the
  * bits are those of the serial solve (tridiagSolver with jump 1,
  * UVZSolver_multithread.cu:203-262, via the FS rows of Sediment_Transport.cu:
@@ -526,7 +528,7 @@ template <bool isJ>
 __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, double s_start, int ketdinh)
 {
     const HdDevG& d = dev_view(dv);
-    constexpr int RW = 512;
+    constexpr int RW = 256;
     __shared__ double sA[RW], sB[RW], sC[RW], sD[RW]; /* rows; factors replace C / D, x replaces A */
     __shared__ double sV[2]; /* clamped x of the first and last row */
     const int N = d.N, M = d.M, W = d.W;
