@@ -248,6 +248,28 @@ def test_long_runs_bitwise():
     assert_same_state(o, g, "channel 300x60, 20 steps")
 
 
+def test_long_sediment_runs_bitwise():
+    """A straight river 700 x 60 along i with sediment and bed change from
+    t = 0: column runs of about 700 cells, so the speculative sediment solve
+    along columns takes three or more 256-row chunks and the UZ sweep three
+    510-row chunks, against the checker bit for bit."""
+    case = synth.river_case(N=700, M=60, hours=1.0, amp=0.0, wmin=20.0, wmax=30.0, n_islands=0,
+                            q_mean=300.0, q_amp=0.0, tide_amp=0.2, sediment=True, name="straight_700x60_sed")
+    case.sediment_start = 0.0
+    case.bed_change_start = 10.0
+    case.bed_mode = BED_EVERY_STEP
+    o, g = pair(case)
+    wet_rows = int((o.get("khouot")[:, 30] == 0).sum())
+    assert wet_rows > 600, wet_rows
+    op = case.opts()
+    t = 0.0
+    for k in range(12):
+        tn = o.step(t, op)
+        g.step(t, op)
+        t = tn
+    assert_same_state(o, g, "straight river 700x60 with sediment, 12 steps")
+
+
 def test_full_river_bitwise():
     """The 980 x 1048 river itself, a few steps, against the checker."""
     oracle.set_threads(min(16, len(os.sched_getaffinity(0))))
