@@ -112,6 +112,43 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def roofline(kt, wet_mean, traffic_file=None):
+    """The roofline object of the dominant kernel from per-label HIP-event
+    totals kt = {label: (total ms, launches)} and the mean wet cells per
+    step.  The two sweeps of one kernel function (labels *_uz and *_vz)
+    count as one kernel: each alone is about as long per step as the state
+    copy's two launches, so a per-label pick would flip between runs.  The
+    traffic is the launch-weighted mean of the labels' HBM bytes per launch
+    in traffic_file (profiles/traffic.json), or None."""
+    groups = {}
+    for k, (ms, n) in kt.items():
+        g_name = k[:-3] if k.endswith(("_uz", "_vz")) else k
+        groups.setdefault(g_name, []).append((k, ms, n))
+    dom_name, members = max(groups.items(), key=lambda kv: sum(ms for _, ms, _ in kv[1]))
+    dom_ms = sum(ms for _, ms, _ in members)
+    dom_n = sum(n for _, _, n in members)
+    avg_s = dom_ms / dom_n * 1e-3
+    per_cells = {KERNEL_BYTES.get(k) for k, _, _ in members}
+    per_cell = per_cells.pop() if len(per_cells) == 1 else None
+    roof = {"kernel": dom_name, "labels": sorted(k for k, _, _ in members), "bound": "hbm",
+            "unit": "GB/s", "peak": HBM_PEAK / 1e9, "avg_launch_us": avg_s * 1e6, "launches": dom_n}
+    if per_cell is not None:
+        bytes_launch = per_cell * wet_mean
+        roof.update(achieved=bytes_launch / avg_s / 1e9, frac=bytes_launch / avg_s / HBM_PEAK,
+                    bytes_per_wet_cell=per_cell, algorithmic_bytes_per_launch=bytes_launch)
+    else:
+        roof.update(achieved=None, frac=None)
+    traffic = None
+    tf = traffic_file or os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        if all(k in tj for k, _, _ in members):
+            traffic = round(sum(tj[k] * n for k, _, n in members) / dom_n)
+    roof["traffic"] = traffic
+    return roof
+
+
 def cpu_baseline(case, budget_s, threads):
     """The CPU checker (same FP64 method, OpenMP over independent lines) on
     the steps of the same run from t = 0 that fit in about `budget_s`
@@ -229,36 +266,7 @@ def main():
     kt = g.kernel_times()
     g.profile(False)
     wet_mean = wet_steps / a.steps
-    # The two sweeps of one kernel function (labels *_uz and *_vz) count as
-    # one kernel: each alone is about as long per step as the state copy's
-    # two launches, so a per-label pick would flip between runs.
-    groups = {}
-    for k, (ms, n) in kt.items():
-        g_name = k[:-3] if k.endswith(("_uz", "_vz")) else k
-        groups.setdefault(g_name, []).append((k, ms, n))
-    dom_name, members = max(groups.items(), key=lambda kv: sum(ms for _, ms, _ in kv[1]))
-    dom_ms = sum(ms for _, ms, _ in members)
-    dom_n = sum(n for _, _, n in members)
-    avg_s = dom_ms / dom_n * 1e-3
-    per_cells = {KERNEL_BYTES.get(k) for k, _, _ in members}
-    per_cell = per_cells.pop() if len(per_cells) == 1 else None
-    roof = {"kernel": dom_name, "labels": sorted(k for k, _, _ in members), "bound": "hbm",
-            "unit": "GB/s", "peak": HBM_PEAK / 1e9, "avg_launch_us": avg_s * 1e6, "launches": dom_n}
-    if per_cell is not None:
-        bytes_launch = per_cell * wet_mean
-        roof.update(achieved=bytes_launch / avg_s / 1e9, frac=bytes_launch / avg_s / HBM_PEAK,
-                    bytes_per_wet_cell=per_cell, algorithmic_bytes_per_launch=bytes_launch)
-    else:
-        roof.update(achieved=None, frac=None)
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            tj = json.load(f)
-        if all(k in tj for k, _, _ in members):
-            # launch-weighted mean of the labels' HBM bytes per launch
-            traffic = round(sum(tj[k] * n for k, _, n in members) / dom_n)
-    roof["traffic"] = traffic
+    roof = roofline(kt, wet_mean)
     kernels = {k: round(ms / n * 1e3, 2) for k, (ms, n) in kt.items()}
 
     cpu = None
