@@ -208,8 +208,9 @@ int hd_set_launch(hd_ctx* c, int rb_mult, int bw);
  * parameters) on one device, stepped together so that every kernel of a step
  * is ONE launch over all of them (grid.z indexes the case; the packed wet-run
  * tables of all cases feed one solver launch).  This has no counterpart in
- * the reference, which runs one case per process (rep.py:145-173); it is the
- * batching that the SURVEY's ensemble workload calls for.  Members are
+ * the reference, which runs one case per process (rep.py:145-173); it serves
+ * ensembles of many scenarios (e.g. 64 rivers of 256-1024 cells a side), far
+ * more work per launch than one case.  Members are
  * ordinary contexts, created, uploaded and hd_init'ed as usual and read back
  * with hd_download; they must share the device and dT, be at the same time
  * and have taken only whole steps (no pending half-step).  A member may not
