@@ -32,6 +32,34 @@ namespace {
 typedef std::vector<double> vd;
 typedef std::vector<int> vi;
 
+/* Planted mistakes for the tests that show a pin catches the mistake it is
+ * meant to catch (or_plant): 0 = none (always, outside those tests).  Each
+ * code changes one operand, sign or index of one restated kernel. */
+enum Plant {
+    PL_NONE = 0,
+    PL_UZ_ADV_INDEX = 1,     /* uz_preindex: central advection difference u[j+1] - u[j] */
+    PL_VZ_CORIOLIS_SIGN = 2, /* vz_preindex: Coriolis term with the wrong sign */
+    PL_US_ZGRAD_INDEX = 3,   /* u_solver: surface slope z[i] - z[i-1] */
+    PL_VS_VISC_DROP = 4,     /* v_solver: the x-part of the viscous term dropped */
+    PL_NORM_COUNT = 5,       /* normalize: neighbour count starts at 3 */
+    PL_WET_OPENS_LAND = 6,   /* reset_states_h: wetting also opens khouot == 2 */
+    PL_QB_AVG_INDEX = 7,     /* calculate_qb: u averaged over t_u[pos], t_u[pos-1] */
+    PL_BED_DIV_SIGN = 8,     /* bed_load: x-divergence of Qbx with the wrong sign */
+    PL_CHS_EROSION = 9,      /* source_chs: erosion excess divided by tob */
+    PL_NCHS_GAMAC = 10,      /* source_nchs: gamac's linear coefficient dropped */
+    PL_HESOK_DEPTH = 11,     /* heso_k: Kx from the node depths of the other face */
+    PL_VTH_AVG = 12,         /* find_vth: u averaged over t_u[pos], t_u[pos-1] */
+    PL_MARGIN_EXTRAP = 13,   /* margins_u: open-edge extrapolation 2 t[f] - t[f+2] */
+    PL_UZ_FRICTION = 14,     /* uz_preindex: friction with |V|^2 instead of |V| */
+    PL_UZ_D2_INDEX = 15,     /* uz_preindex: y-flux difference shifted by one column */
+    PL_ABCD_C1_SIGN = 16,    /* calculate_abcd: c1's continuity term with the wrong sign */
+    PL_WET_NO_CASCADE = 17,  /* reset_states_h: an opened cell does not pass the wetting on */
+    PL_DRY_FACE = 18,        /* reset_states_h: drying zeroes t_u[i+1] instead of t_u[i-1] */
+    PL_F4_FACTOR = 19,       /* uz_solve: surface-slope coefficient g dT / (2 dX) instead of g dT / dX */
+};
+int g_plant = PL_NONE;
+inline bool planted(int k) { return g_plant == k; }
+
 struct Oracle {
     hd_params P;
     int N, M, SL, W, series_len, total_time;
@@ -84,6 +112,11 @@ const FieldRef kFields[] = {
     /* scratch rows of the last tridiagonal assembly (reference layout) */
     {"AA", &Oracle::AA, nullptr}, {"BB", &Oracle::BB, nullptr}, {"CC", &Oracle::CC, nullptr},
     {"DD", &Oracle::DD, nullptr}, {"x", &Oracle::x, nullptr}, {"SN", nullptr, &Oracle::SN},
+    /* per-line assembly scratch (UZ: line j at j*(N+2); VZ: line i at i*(M+2)) */
+    {"a1", &Oracle::a1, nullptr}, {"b1", &Oracle::b1, nullptr}, {"c1", &Oracle::c1, nullptr},
+    {"d1", &Oracle::d1, nullptr}, {"a2", &Oracle::a2, nullptr}, {"c2", &Oracle::c2, nullptr},
+    {"d2", &Oracle::d2, nullptr}, {"f1", &Oracle::f1, nullptr}, {"f2", &Oracle::f2, nullptr},
+    {"f3", &Oracle::f3, nullptr}, {"f5", &Oracle::f5, nullptr},
 };
 
 /* ------------------------------------------------------------------ params */
@@ -440,8 +473,9 @@ void uz_preindex(Oracle& o, int i, int j)
     double p = 0.0, q = 0.0;
     double vtb = (v[i * W + j - 1] + v[i * W + j] + v[(i + 1) * W + j - 1] + v[(i + 1) * W + j]) * 0.25;
     f1[i] = P.dTchia2dX * u[i * W + j] + VIS[i * W + j] * P.dT / P.dXbp;
-    f2[i] = -(2 + Kx1[i * W + j] * P.dT * std::sqrt(u[i * W + j] * u[i * W + j] + vtb * vtb) / Htdu[i * W + j]
-              + (2 * P.dT * VIS[i * W + j]) / P.dXbp);
+    double speed = std::sqrt(u[i * W + j] * u[i * W + j] + vtb * vtb);
+    if (planted(PL_UZ_FRICTION)) speed = speed * speed;
+    f2[i] = -(2 + Kx1[i * W + j] * P.dT * speed / Htdu[i * W + j] + (2 * P.dT * VIS[i * W + j]) / P.dXbp);
     f3[i] = P.dT * VIS[i * W + j] / P.dXbp - P.dTchia2dX * u[i * W + j];
     if (Hm[i * W + j - 1] <= P.H_TINH) {
         if (vtb < 0) {
@@ -455,7 +489,7 @@ void uz_preindex(Oracle& o, int i, int j)
                 q = (u[i * W + j] - 2 * u[i * W + j - 1] + u[i * W + j - 2]) / P.dYbp;
             }
         } else {
-            p = vtb * (u[i * W + j + 1] - u[i * W + j - 1]) / P.dY2;
+            p = vtb * (u[i * W + j + 1] - u[i * W + j - (planted(PL_UZ_ADV_INDEX) ? 0 : 1)]) / P.dY2;
             q = (u[i * W + j + 1] - 2 * u[i * W + j] + u[i * W + j - 1]) / P.dYbp;
         }
     }
@@ -463,7 +497,8 @@ void uz_preindex(Oracle& o, int i, int j)
             - P.dT * (P.Windx - Tsxw[i * W + j]) / Htdu[i * W + j];
     c2[i] = P.dTchia2dX * Htdu[i * W + j];
     a2[i] = -P.dTchia2dX * Htdu[(i - 1) * W + j];
-    d2[i] = z[i * W + j] - P.dTchia2dY * (Htdv[i * W + j] * v[i * W + j] - Htdv[i * W + j - 1] * v[i * W + j - 1]);
+    int jd = planted(PL_UZ_D2_INDEX) ? j + 1 : j;
+    d2[i] = z[i * W + j] - P.dTchia2dY * (Htdv[i * W + jd] * v[i * W + jd] - Htdv[i * W + jd - 1] * v[i * W + jd - 1]);
 }
 
 /* _vzSolver_calculate_preindex, UVZSolver_multithread.cu:295-350, cell (i, j). */
@@ -497,7 +532,7 @@ void vz_preindex(Oracle& o, int i, int j)
             p = (v[(i + 1) * W + j] - 2 * v[i * W + j] + v[(i - 1) * W + j]) / P.dXbp;
         }
     }
-    f5[j] = -2 * v[i * W + j] + P.dT * q + P.dT * P.CORIOLIS_FORCE * utb - P.dT * VIS[i * W + j] * p
+    f5[j] = -2 * v[i * W + j] + P.dT * q + (planted(PL_VZ_CORIOLIS_SIGN) ? -1 : 1) * P.dT * P.CORIOLIS_FORCE * utb - P.dT * VIS[i * W + j] * p
             - P.dT * (P.Windy - Tsyw[i * W + j]) / Htdv[i * W + j];
     c2[j] = P.dTchia2dY * Htdv[i * W + j];
     a2[j] = -P.dTchia2dY * Htdv[i * W + j - 1];
@@ -517,7 +552,7 @@ void calculate_abcd(Oracle& o, int line, int first, int last, double f4, int sw,
     if (last - 1 > first) {
         for (int j = first; j < last; ++j) {
             a1[j] = f4 - f1[j] / a2[j];
-            c1[j] = -f4 - (f3[j] / c2[j + 1]);
+            c1[j] = -f4 - (planted(PL_ABCD_C1_SIGN) ? -1 : 1) * (f3[j] / c2[j + 1]);
             b1[j] = f2[j] - (f1[j] * c2[j] / a2[j]) - (f3[j] * a2[j + 1] / c2[j + 1]);
             d1[j] = f5[j] - (f1[j] * d2[j] / a2[j]) - (f3[j] * d2[j + 1] / c2[j + 1]);
         }
@@ -747,7 +782,7 @@ void uz_solve(Oracle& o, int sidx, int eidx)
 {
     const hd_params& P = o.P;
     int N = o.N, SL = o.SL, sw = N + 2, tw = 2 * N + 1;
-    double f4 = 2 * P.g * P.dTchia2dX;
+    double f4 = (planted(PL_F4_FACTOR) ? 1 : 2) * P.g * P.dTchia2dX;
 #pragma omp parallel for schedule(dynamic, 4)
     for (int j = sidx; j <= eidx; ++j) {
         int nseg = o.mocj[j];
@@ -828,7 +863,7 @@ void normalize(Oracle& o, bool isU)
             int p = i * W + j;
             double r = val[p];
             if (val[p] != 0) {
-                int count = 2;
+                int count = planted(PL_NORM_COUNT) ? 3 : 2;
                 double nb = val[p - close] + val[p + close];
                 if (o.khouot[p - far] == 0) { nb += val[p - far]; count++; }
                 if (o.khouot[p + far] == 0) { nb += val[p + far]; count++; }
@@ -872,7 +907,7 @@ void v_solver(Oracle& o, int row, int col)
     }
     q = P.HaiChiadT * v[row * W + col] - q - P.CORIOLIS_FORCE * t_utb;
     q = (q - P.g * (z[row * W + col + 1] - z[row * W + col]) / P.dY
-         + VIS[row * W + col] * (tmp + (v[row * W + col + 1] - 2 * v[row * W + col] + v[row * W + col - 1]) / P.dYbp))
+         + VIS[row * W + col] * ((planted(PL_VS_VISC_DROP) ? 0.0 : tmp) + (v[row * W + col + 1] - 2 * v[row * W + col] + v[row * W + col - 1]) / P.dYbp))
         + (P.Windy - Tsyw[row * W + col]) / Htdv[row * W + col];
     o.t_v[row * W + col] = q / p;
 }
@@ -907,7 +942,8 @@ void u_solver(Oracle& o, int row, int col)
         }
     }
     q = P.HaiChiadT * u[row * W + col] - q + P.CORIOLIS_FORCE * t_vtb;
-    q = (q - P.g * (z[(row + 1) * W + col] - z[row * W + col]) / P.dX
+    int zr = planted(PL_US_ZGRAD_INDEX) ? row - 1 : row;
+    q = (q - P.g * (z[(zr + 1) * W + col] - z[zr * W + col]) / P.dX
          + VIS[row * W + col] * (tmp + (u[(row + 1) * W + col] - 2 * u[row * W + col] + u[(row - 1) * W + col]) / P.dXbp))
         + (P.Windx - Tsxw[row * W + col]) / Htdu[row * W + col];
     o.t_u[row * W + col] = q / p;
@@ -973,7 +1009,7 @@ void margins_u(Oracle& o, int sidx, int eidx)
             bool br1 = (first > 2) || ((first == 2) && ((std::fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - o.P.NANGDAY) < kEpsilon));
             bool br2 = (last < N) || ((last == N) && (std::fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - o.P.NANGDAY) < kEpsilon));
             if (br1) t_u[(first - 1) * W + j] = 0;
-            else t_u[(first - 1) * W + j] = 2 * t_u[first * W + j] - t_u[(first + 1) * W + j];
+            else t_u[(first - 1) * W + j] = 2 * t_u[first * W + j] - t_u[(first + (planted(PL_MARGIN_EXTRAP) ? 2 : 1)) * W + j];
             if (br2) t_u[last * W + j] = 0;
             else t_u[last * W + j] = 2 * t_u[(last - 1) * W + j] - t_u[(last - 2) * W + j];
         }
@@ -998,26 +1034,33 @@ void reset_states_h(Oracle& o)
            *FS = o.FS.data();
     const double* z = o.z.data();
     int* kh = o.khouot.data();
+    std::vector<int> opened; /* PL_WET_NO_CASCADE only */
+    auto opened_here = [&](int p) {
+        for (int q : opened) if (q == p) return true;
+        return false;
+    };
     for (int i = 1; i <= o.N; ++i) {
         for (int j = 2; j <= M; ++j) {
+            if (planted(PL_WET_NO_CASCADE) && opened_here(i * W + j)) continue;
             if (t_z[i * W + j] > z[i * W + j]) {
-                if (kh[i * W + j - 1] == 1) {
+                if (kh[i * W + j - 1] == 1 || (planted(PL_WET_OPENS_LAND) && kh[i * W + j - 1] == 2)) {
                     t_z[i * W + j - 1] = t_z[i * W + j];
                     Hm[i * W + j - 1] = htaiz[i * W + j - 1] + t_z[i * W + j];
                     kh[i * W + j - 1] = 0;
                     FS[i * W + j - 1] = FS[i * W + j];
                 }
-                if (kh[i * W + j + 1] == 1) {
+                if (kh[i * W + j + 1] == 1 || (planted(PL_WET_OPENS_LAND) && kh[i * W + j + 1] == 2)) {
                     t_z[i * W + j + 1] = t_z[i * W + j];
                     Hm[i * W + j + 1] = htaiz[i * W + j + 1] + t_z[i * W + j];
                     kh[i * W + j + 1] = 0;
                     FS[i * W + j + 1] = FS[i * W + j];
+                    if (planted(PL_WET_NO_CASCADE)) opened.push_back(i * W + j + 1);
                 }
             }
         }
         for (int j = 2; j <= M; ++j) {
             if ((kh[i * W + j] == 0) && (Hm[i * W + j] <= P.H_TINH)) {
-                t_u[(i - 1) * W + j] = 0;
+                t_u[(planted(PL_DRY_FACE) ? i + 1 : i - 1) * W + j] = 0;
                 t_u[i * W + j] = 0;
                 t_v[i * W + j - 1] = 0;
                 t_v[i * W + j] = 0;
@@ -1090,7 +1133,7 @@ double source_chs(const hd_params& P, double wsm, double Zf, double fs, double f
     double Pd, beta, Cb;
     double tob = P.ro * fw * (vth * vth);
     if (h_moi > P.ghtoe) toee = P.Toe * (1 + P.hstoe * ((h_moi - P.ghtoe)));
-    if (tob > toee) S = P.Mbochat / P.ros * (tob - toee) / toee;
+    if (tob > toee) S = P.Mbochat / P.ros * (tob - toee) / (planted(PL_CHS_EROSION) ? tob : toee);
     else if (tob < todd) {
         Pd = 1 - (tob / todd);
         beta = 1 + (Zf / (1.25 + 4.75 * hd_pow(Pd, 2.5)));
@@ -1103,7 +1146,7 @@ double source_chs(const hd_params& P, double wsm, double Zf, double fs, double f
 /* source_nchs, Sediment_Transport.cu:52-60. */
 double source_nchs(const hd_params& P, double wsm, double Zf, double Dxr, double Ufr, double Uf, double h_moi, double fs)
 {
-    double gamac = 0.434 + 5.975 * Zf + 2.888 * Zf * Zf;
+    double gamac = 0.434 + (planted(PL_NCHS_GAMAC) ? 0.0 : 5.975) * Zf + 2.888 * Zf * Zf;
     double Tx = std::fmax(0.0, (Uf * Uf - Ufr * Ufr) / (Ufr * Ufr));
     double Cac = 0.015 * P.dm * hd_pow(Tx, 1.5) * hd_pow(Dxr, (-0.3)) / (0.05 * h_moi);
     return wsm * (Cac - gamac * fs);
@@ -1117,7 +1160,7 @@ void find_vth(Oracle& o)
         for (int j = 2; j <= o.M; ++j) {
             int pos = i * W + j;
             if ((o.htaiz[pos] > o.P.NANGDAY) && (o.H_moi[pos] > o.P.H_TINH)) {
-                double ut = (o.t_u[pos] + o.t_u[pos - W]) * 0.5;
+                double ut = (o.t_u[pos] + o.t_u[pos - (planted(PL_VTH_AVG) ? 1 : W)]) * 0.5;
                 double vt = (o.t_v[pos] + o.t_v[pos - 1]) * 0.5;
                 o.VTH[pos] = std::sqrt(ut * ut + vt * vt);
             } else o.VTH[pos] = 0;
@@ -1137,7 +1180,7 @@ void heso_k(Oracle& o)
             double Cz;
             if (o.Htdu[pos] > 0) {
                 Cz = 18 * hd_log(12 * o.Htdu[pos] / P.Ks);
-                o.Kx[pos] = 5.93 * std::sqrt(P.g) * (o.h[pos - 1] + o.h[pos]) * 0.25 * std::fabs(o.t_u[pos]) / Cz;
+                o.Kx[pos] = 5.93 * std::sqrt(P.g) * (o.h[pos - (planted(PL_HESOK_DEPTH) ? W : 1)] + o.h[pos]) * 0.25 * std::fabs(o.t_u[pos]) / Cz;
                 o.Kx[pos] = std::fmin(100.0, std::fmax(5.0, o.Kx[pos]));
             }
             if (o.Htdv[pos] > 0) {
@@ -1344,7 +1387,7 @@ void calculate_qb(Oracle& o, int qb_mode)
                 if (o.H_moi[pos] > P.ghtoe) Toee = P.Toe * (1 + P.hstoe * ((o.H_moi[pos] - P.ghtoe)));
                 if (Tob > Toee) Tx = Tob / Toee - 1;
                 Qbx[pos] = 0.053 * hd_pow((P.Sx - 1) * P.g, 0.5) * hd_pow(P.dm, 1.5) * hd_pow(Tx, 2.1) * hd_pow(P.Dxr, -0.3)
-                           * (o.t_u[pos] + o.t_u[pos - W]) * 0.5 / o.VTH[pos];
+                           * (o.t_u[pos] + o.t_u[pos - (planted(PL_QB_AVG_INDEX) ? 1 : W)]) * 0.5 / o.VTH[pos];
                 Qby[pos] = 0.053 * hd_pow((P.Sx - 1) * P.g, 0.5) * hd_pow(P.dm, 1.5) * hd_pow(Tx, 2.1) * hd_pow(P.Dxr, -0.3)
                            * (o.t_v[pos] + o.t_v[pos - 1]) * 0.5 / o.VTH[pos];
             } else {
@@ -1388,7 +1431,7 @@ void bed_load(Oracle& o, bool ketdinh, int sidx, int eidx)
                 } else {
                     if (kh[pos + W] == 1) {
                         if (t_u[1]) p = (3 * Qbx[pos] - 4 * Qbx[pos - W] + Qbx[pos - 2 * W]) / P.dX2;
-                    } else p = (Qbx[pos + W] - Qbx[pos - W]) / P.dX2;
+                    } else p = (planted(PL_BED_DIV_SIGN) ? -1 : 1) * (Qbx[pos + W] - Qbx[pos - W]) / P.dX2;
                 }
                 if (kh[pos - 1] == 1) {
                     if (t_v[pos] < 0) q = (-3 * Qby[pos] + 4 * Qby[pos + 1] - Qby[pos + 2]) / P.dY2;
@@ -1677,6 +1720,78 @@ int or_set_threads(int n)
     (void)n;
     return 1;
 #endif
+}
+
+/* One restated kernel on the current state, for the tests that pin each
+ * against closed forms and for the per-kernel GPU parity tests.  Arguments by
+ * kernel (a, b, c integers, t a time):
+ *   uz_preindex / vz_preindex (a = i, b = j): one cell, into f1..f5, a2, c2, d2
+ *   uz_solve / vz_solve (a = first line, b = last line): preindex, abcd,
+ *       matrix_coeff, tridiag and extraction of those lines
+ *   u_solver / v_solver (a = i, b = j): one explicit cell into t_u / t_v
+ *   solve_u / solve_v, margins_u / margins_v (a, b = line range)
+ *   normalize (a = 1: t_u, 0: t_v), update_h_moi, reset_states_h,
+ *   reset_states_v, update_uvz (a = kenhhepd, b = kenhhepng), find_limits,
+ *   htuongdoi, find_vth, heso_k, calculate_qb (a = qb_mode),
+ *   bed_load (a = ketdinh, b = sidx, c = eidx),
+ *   sediment_j / sediment_i (a = ketdinh, t; the scan, solve, extraction and
+ *       update of the FS sweep, bed-change start taken from c as seconds),
+ *   boundary (t: Update_Boundary_Value), channel_boundary (t). */
+int or_kernel(void* h, const char* name, int a, int b, int c, double t)
+{
+    Oracle& o = *(Oracle*)h;
+    o.err = 0;
+    std::string n(name);
+    if (n == "uz_preindex") uz_preindex(o, a, b);
+    else if (n == "vz_preindex") vz_preindex(o, a, b);
+    else if (n == "uz_solve") uz_solve(o, a, b);
+    else if (n == "vz_solve") vz_solve(o, a, b);
+    else if (n == "u_solver") u_solver(o, a, b);
+    else if (n == "v_solver") v_solver(o, a, b);
+    else if (n == "solve_u") solve_u(o, a, b);
+    else if (n == "solve_v") solve_v(o, a, b);
+    else if (n == "margins_u") margins_u(o, a, b);
+    else if (n == "margins_v") margins_v(o, a, b);
+    else if (n == "normalize") normalize(o, a != 0);
+    else if (n == "update_h_moi") update_h_moi(o);
+    else if (n == "reset_states_h") reset_states_h(o);
+    else if (n == "reset_states_v") reset_states_v(o);
+    else if (n == "update_uvz") update_uvz(o, a, b);
+    else if (n == "find_limits") find_limits(o);
+    else if (n == "htuongdoi") htuongdoi(o);
+    else if (n == "find_vth") find_vth(o);
+    else if (n == "heso_k") heso_k(o);
+    else if (n == "calculate_qb") calculate_qb(o, a);
+    else if (n == "bed_load") bed_load(o, a != 0, b, c);
+    else if (n == "sediment_j") sediment_j(o, t, (double)c, a != 0);
+    else if (n == "sediment_i") sediment_i(o, t, (double)c, a != 0);
+    else if (n == "boundary") update_boundary_value(o, t);
+    else if (n == "channel_boundary") channel_boundaries(o, t);
+    else { o.msg = "unknown kernel " + n; return HD_ENAME; }
+    return o.err;
+}
+
+/* source_chs / source_nchs (Sediment_Transport.cu:12-60) with the case's
+ * parameters, for closed-form checks. */
+double or_source_chs(void* h, double wsm, double Zf, double fs, double fw, double vth, double h_moi)
+{
+    return source_chs(((Oracle*)h)->P, wsm, Zf, fs, fw, vth, h_moi);
+}
+
+double or_source_nchs(void* h, double wsm, double Zf, double Uf, double h_moi, double fs)
+{
+    const hd_params& P = ((Oracle*)h)->P;
+    return source_nchs(P, wsm, Zf, P.Dxr, P.Ufr, Uf, h_moi, fs);
+}
+
+/* Plant mistake `code` (enum Plant) in the restated kernels, 0 to clear;
+ * returns the previous code.  Only the tests that show a pin catches its
+ * mistake set it. */
+int or_plant(int code)
+{
+    int prev = g_plant;
+    g_plant = code;
+    return prev;
 }
 
 /* hd_math entry points, for accuracy tests against libm. */

@@ -65,6 +65,14 @@ def lib():
         L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_hd_pow.restype = ctypes.c_double
         L.or_hd_pow.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.or_kernel.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        d = ctypes.c_double
+        L.or_source_chs.restype = d
+        L.or_source_chs.argtypes = [vp, d, d, d, d, d, d]
+        L.or_source_nchs.restype = d
+        L.or_source_nchs.argtypes = [vp, d, d, d, d, d]
+        L.or_plant.restype = ctypes.c_int
+        L.or_plant.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -151,6 +159,46 @@ class Oracle:
 
     def wet_count(self):
         return int(lib().or_wet_count(self.h))
+
+    def kernel(self, name, a=0, b=0, c=0, t=0.0):
+        """Run one restated kernel on the current state (or_kernel)."""
+        self._check(lib().or_kernel(self.h, name.encode(), int(a), int(b), int(c), float(t)))
+
+    def scratch(self, name):
+        """A per-line assembly scratch array (a1..f5: (L+3)^2 doubles;
+        AA..DD, x: (L+3)(2L+4) doubles, L = max(N, M))."""
+        L = max(self.N, self.M)
+        n = (L + 3) * (2 * L + 4) if name in ("AA", "BB", "CC", "DD", "x") else (L + 3) * (L + 3)
+        return self.get(name, like=np.zeros(n))
+
+    def source_chs(self, wsm, Zf, fs, fw, vth, h_moi):
+        return float(lib().or_source_chs(self.h, wsm, Zf, fs, fw, vth, h_moi))
+
+    def source_nchs(self, wsm, Zf, Uf, h_moi, fs):
+        return float(lib().or_source_nchs(self.h, wsm, Zf, Uf, h_moi, fs))
+
+
+class planted:
+    """Context manager: plant mistake `code` (hydist_oracle.cpp enum Plant)
+    in the checker's kernels for the duration of a block, so a test can show
+    that its pin fails on that mistake."""
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        self.prev = lib().or_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_plant(self.prev)
+        return False
+
+
+PLANTS = dict(UZ_ADV_INDEX=1, VZ_CORIOLIS_SIGN=2, US_ZGRAD_INDEX=3, VS_VISC_DROP=4, NORM_COUNT=5,
+              WET_OPENS_LAND=6, QB_AVG_INDEX=7, BED_DIV_SIGN=8, CHS_EROSION=9, NCHS_GAMAC=10,
+              HESOK_DEPTH=11, VTH_AVG=12, MARGIN_EXTRAP=13, UZ_FRICTION=14, UZ_D2_INDEX=15,
+              ABCD_C1_SIGN=16, WET_NO_CASCADE=17, DRY_FACE=18, F4_FACTOR=19)
 
 
 def thomas(A, B, C, D):

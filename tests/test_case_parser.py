@@ -32,8 +32,9 @@ def test_float_case_rejected():
 
 
 def test_reference_headers_golden():
-    """Values of the reference's case headers (evaluated once from the
-    header text and committed as data)."""
+    """The parser against the reference's case headers as g++ evaluates them
+    (constexpr of each expression as written; tests/golden/make_case_headers.py
+    wrote the fixture)."""
     with open(os.path.join(HERE, "golden", "case_headers.json")) as f:
         golden = json.load(f)
     ref = "/root/reference"
