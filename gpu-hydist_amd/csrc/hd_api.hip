@@ -1423,4 +1423,214 @@ int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const d
     return rc;
 }
 
+
+/* ------------------------------------------------------------ per-kernel entry points */
+
+/* The library's kernel(s) for one reference kernel, on the current state of a
+ * context between whole steps, waited for before returning.  For unit parity
+ * against the checker; the step entry points are the product path. */
+static int unit_begin(hd_ctx* c, Exec& x)
+{
+    if (!c) return HD_EINVAL;
+    if (c->state() != 0) return fail(c, HD_ESTATE, "per-kernel entry point inside a half-step");
+    HIPCHK(c, hipSetDevice(c->device));
+    x = exec_of(c);
+    return HD_OK;
+}
+
+static int unit_end(hd_ctx* c, bool mask_changed)
+{
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mask_changed) {
+        /* the next step re-derives its active tiles over the whole grid */
+        c->settled = false;
+        c->sed_settled = false;
+        c->bed_settled = false;
+    }
+    return check_flags(c);
+}
+
+/* rows of the line-major buffers (line stride 2*max(N,M)+4) into a caller
+ * array with the reference's line stride */
+static int rows_out(hd_ctx* c, double* dev, double* host, int nlines, int stride)
+{
+    if (!host) return HD_OK;
+    int L = c->N > c->M ? c->N : c->M;
+    size_t tw = 2 * (size_t)L + 4;
+    int w = (size_t)stride < tw ? stride : (int)tw;
+    HIPCHK(c, hipMemcpy2D(host, (size_t)stride * 8, dev, tw * 8, (size_t)w * 8, nlines, hipMemcpyDeviceToHost));
+    return HD_OK;
+}
+
+int hd_k_Onetime_init(hd_ctx* c)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_onetime_init(x.b(), x.stream);
+    return unit_end(c, true);
+}
+
+int hd_k_Find_Calculation_limits(hd_ctx* c)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_find_limits(x.b(), x.stream);
+    if ((rc = unit_end(c, true))) return rc;
+    return compute_tiles(c);
+}
+
+int hd_k_Htuongdoi(hd_ctx* c)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_htuongdoi(x.b(), x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_preprocess_data(hd_ctx* c)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_preprocess(x.b(), x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_Update_Boundary_Value(hd_ctx* c, double t, int analytic)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_step_opts o{};
+    o.boundary_mode = analytic ? HD_BND_ANALYTIC : HD_BND_SERIES;
+    if ((rc = check_series(c, t, o))) return rc;
+    set_time(x, t - c->P.dT * 0.5);          /* the boundary kernel advances it by dT/2 */
+    hd_launch_boundary_t(x.b(), analytic ? 1 : 0, x.stream);
+    return unit_end(c, false);
+}
+
+static int k_sweep(hd_ctx* c, int isU, int shift)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_adi_spec(x.b(), isU, shift, isU ? x.run_blocks_u : x.run_blocks_v, x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_UZSolver(hd_ctx* c, int shift) { return k_sweep(c, 1, shift); }
+int hd_k_VZSolver(hd_ctx* c, int shift) { return k_sweep(c, 0, shift); }
+
+static int k_matrix(hd_ctx* c, int isU, int shift, double* AA, double* BB, double* CC, double* DD, int* SN)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_adi_coef(x.b(), isU, shift, x.stream);
+    if ((rc = unit_end(c, false))) return rc;
+    int nl = (isU ? c->M : c->N) + 3, stride = 2 * (isU ? c->N : c->M) + 1;
+    double* dev[] = {c->d.AA, c->d.BB, c->d.CC, c->d.DD};
+    double* host[] = {AA, BB, CC, DD};
+    for (int k = 0; k < 4; ++k)
+        if ((rc = rows_out(c, dev[k], host[k], nl, stride))) return rc;
+    int L = c->N > c->M ? c->N : c->M;
+    if (SN) HIPCHK(c, hipMemcpy(SN, c->d.seg_sn, (size_t)(L + 3) * c->SL * 4, hipMemcpyDeviceToHost));
+    return HD_OK;
+}
+
+int hd_k_UZSolver_calculate_matrix_coeff(hd_ctx* c, int shift, double* AA, double* BB, double* CC, double* DD,
+                                         int* SN)
+{
+    return k_matrix(c, 1, shift, AA, BB, CC, DD, SN);
+}
+
+int hd_k_VZSolver_calculate_matrix_coeff(hd_ctx* c, int shift, double* AA, double* BB, double* CC, double* DD,
+                                         int* SN)
+{
+    return k_matrix(c, 0, shift, AA, BB, CC, DD, SN);
+}
+
+int hd_k_Normalize(hd_ctx* c, int isU)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_normalize_post(x.b(), isU ? 1 : 0, 0, isU ? 0 : 1, x.bw, 0, x.stream);
+    size_t F = (size_t)(c->N + 3) * (c->M + 3) * 8;
+    /* update_buffer: the smoothed partner back into t_u / t_v */
+    HIPCHK(c, hipMemcpyAsync(isU ? c->d.t_u : c->d.t_v, isU ? c->d.tmp_u : c->d.tmp_v, F, hipMemcpyDeviceToDevice,
+                             x.stream));
+    return unit_end(c, false);
+}
+
+static int k_explicit(hd_ctx* c, int isU)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_explicit(x.b(), isU, x.stream);
+    hd_launch_margins(x.b(), isU, x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_solveV(hd_ctx* c) { return k_explicit(c, 0); }
+int hd_k_solveU(hd_ctx* c) { return k_explicit(c, 1); }
+
+int hd_k_Reset_states(hd_ctx* c, int horizontal, int kenhhepd, int kenhhepng)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    /* update_h_moi, the drying test and the wetting candidates (its smoothing
+     * output in the partner buffer is discarded), the serial wetting pass,
+     * then the drying writes with update_uvz and Htuongdoi */
+    hd_launch_normalize_post(x.b(), horizontal ? 0 : 1, 1, horizontal ? 1 : 0, x.bw, 0, x.stream);
+    hd_launch_wetting(x.b(), horizontal ? 1 : 0, x.wet_mode, x.stream);
+    hd_launch_uvz_dry(x.b(), kenhhepd, kenhhepng, x.bw, 0, x.stream);
+    return unit_end(c, true);
+}
+
+int hd_k_Find_VTH_hesoK(hd_ctx* c)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_vth_hesok(x.b(), 0, x.stream);
+    return unit_end(c, false);
+}
+
+static int k_fs(hd_ctx* c, int which, double t, double s_start, int ketdinh)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    set_time(x, t);
+    hd_launch_fs_run(x.b(), which, s_start, ketdinh, which == 0 ? x.run_blocks_u : x.run_blocks_v, 1, x.stream);
+    hd_launch_update_fs(x.b(), 0, x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_FSj_solve(hd_ctx* c, double t, double s_start, int ketdinh) { return k_fs(c, 0, t, s_start, ketdinh); }
+int hd_k_FSi_solve(hd_ctx* c, double t, double s_start, int ketdinh) { return k_fs(c, 1, t, s_start, ketdinh); }
+
+static int k_fs_rows(hd_ctx* c, int which, double t, double s_start, int ketdinh, double* AA, double* BB,
+                     double* CC, double* DD)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    set_time(x, t);
+    hd_launch_fs_coef(x.b(), which, s_start, ketdinh, x.stream);
+    if ((rc = unit_end(c, false))) return rc;
+    int nl = (which == 0 ? c->M : c->N) + 3, stride = (which == 0 ? c->N : c->M) + 3;
+    double* dev[] = {c->d.AA, c->d.BB, c->d.CC, c->d.DD};
+    double* host[] = {AA, BB, CC, DD};
+    for (int k = 0; k < 4; ++k)
+        if ((rc = rows_out(c, dev[k], host[k], nl, stride))) return rc;
+    return HD_OK;
+}
+
+int hd_k_Scan_FSj(hd_ctx* c, double t, double s_start, int ketdinh, double* AA, double* BB, double* CC, double* DD)
+{
+    return k_fs_rows(c, 0, t, s_start, ketdinh, AA, BB, CC, DD);
+}
+
+int hd_k_Scan_FSi(hd_ctx* c, double t, double s_start, int ketdinh, double* AA, double* BB, double* CC, double* DD)
+{
+    return k_fs_rows(c, 1, t, s_start, ketdinh, AA, BB, CC, DD);
+}
+
+int hd_k_Calculate_Qb(hd_ctx* c, int qb_mode)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    hd_launch_calc_qb(x.b(), qb_mode, 0, x.stream);
+    return unit_end(c, false);
+}
+
+int hd_k_BedLoad(hd_ctx* c, int ketdinh, int sidx, int eidx)
+{
+    Exec x; int rc = unit_begin(c, x); if (rc) return rc;
+    if (sidx < 2 || eidx > c->N || sidx > eidx) return fail(c, HD_EINVAL, "hd_k_BedLoad: rows out of range");
+    hd_launch_bedload(x.b(), ketdinh, sidx, c->N - eidx, 0, x.stream);
+    return unit_end(c, false);
+}
+
 } // extern "C"

@@ -26,6 +26,11 @@ EXPORTS = (
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
     "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
     "hd_z_snapshot", "hd_z_change",
+    "hd_k_Onetime_init", "hd_k_Find_Calculation_limits", "hd_k_Htuongdoi", "hd_k_preprocess_data",
+    "hd_k_Update_Boundary_Value", "hd_k_UZSolver", "hd_k_VZSolver", "hd_k_UZSolver_calculate_matrix_coeff",
+    "hd_k_VZSolver_calculate_matrix_coeff", "hd_k_Normalize", "hd_k_solveV", "hd_k_solveU", "hd_k_Reset_states",
+    "hd_k_Find_VTH_hesoK", "hd_k_FSj_solve", "hd_k_FSi_solve", "hd_k_Scan_FSj", "hd_k_Scan_FSi",
+    "hd_k_Calculate_Qb", "hd_k_BedLoad",
 )
 
 _lib = None
@@ -74,6 +79,26 @@ def load() -> ctypes.CDLL:
         "hd_profile": (ci, [vp, ci]),
         "hd_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
         "hd_thomas_batch": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
+        "hd_k_Onetime_init": (ci, [vp]),
+        "hd_k_Find_Calculation_limits": (ci, [vp]),
+        "hd_k_Htuongdoi": (ci, [vp]),
+        "hd_k_preprocess_data": (ci, [vp]),
+        "hd_k_Update_Boundary_Value": (ci, [vp, cd, ci]),
+        "hd_k_UZSolver": (ci, [vp, ci]),
+        "hd_k_VZSolver": (ci, [vp, ci]),
+        "hd_k_UZSolver_calculate_matrix_coeff": (ci, [vp, ci, vp, vp, vp, vp, vp]),
+        "hd_k_VZSolver_calculate_matrix_coeff": (ci, [vp, ci, vp, vp, vp, vp, vp]),
+        "hd_k_Normalize": (ci, [vp, ci]),
+        "hd_k_solveV": (ci, [vp]),
+        "hd_k_solveU": (ci, [vp]),
+        "hd_k_Reset_states": (ci, [vp, ci, ci, ci]),
+        "hd_k_Find_VTH_hesoK": (ci, [vp]),
+        "hd_k_FSj_solve": (ci, [vp, cd, cd, ci]),
+        "hd_k_FSi_solve": (ci, [vp, cd, cd, ci]),
+        "hd_k_Scan_FSj": (ci, [vp, cd, cd, ci, vp, vp, vp, vp]),
+        "hd_k_Scan_FSi": (ci, [vp, cd, cd, ci, vp, vp, vp, vp]),
+        "hd_k_Calculate_Qb": (ci, [vp, ci]),
+        "hd_k_BedLoad": (ci, [vp, ci, ci, ci]),
         "hd_flow_refresh": (ci, [vp]),
         "hd_clone": (ci, [vp, ctypes.POINTER(vp)]),
         "hd_autotune": (ci, [vp, cd, po, ci, ip, ip, dp]),
@@ -198,6 +223,30 @@ class Solver:
 
     def sync(self):
         self._chk(load().hd_sync(self.h))
+
+    def kernel(self, name, *args):
+        """One reference kernel on the current state: hd_k_<name>(ctx, *args)
+        (per-kernel entry points of include/hydist.h, for unit parity)."""
+        self._chk(getattr(load(), "hd_k_" + name)(self.h, *args))
+
+    def rows(self, name, *args):
+        """hd_k_<name> of a row-assembly kernel (UZSolver_/VZSolver_
+        calculate_matrix_coeff, Scan_FSj, Scan_FSi): its AA, BB, CC, DD in
+        the reference's layout (only the systems' rows are defined) and, for
+        the momentum/continuity rows, SN."""
+        N, M, L = self.N, self.M, max(self.N, self.M)
+        if name.startswith(("UZ", "VZ")):
+            n = (M + 3) * (2 * N + 1) if name.startswith("UZ") else (N + 3) * (2 * M + 1)
+        else:
+            n = (M + 3) * (N + 3)
+        out = [np.full(n, np.nan) for _ in range(4)]
+        ptrs = [a.ctypes.data for a in out]
+        if name.startswith(("UZ", "VZ")):
+            sn = np.zeros((L + 3) * self.params().segment_limit, np.int32)
+            self.kernel(name, *args, *ptrs, sn.ctypes.data)
+            return out + [sn]
+        self.kernel(name, *args, *ptrs)
+        return out
 
     def clone(self):
         """A second Solver holding a copy of this one's device state."""

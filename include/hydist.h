@@ -236,6 +236,72 @@ int hd_thomas_batch(int device, int nsys, const int* off, const int* sn,
                     const double* A, const double* B, const double* C,
                     const double* D, double* x, int nrows);
 
+/* Per-kernel entry points for unit parity.  Each runs the library's kernel(s)
+ * for the reference kernel it is named after on the context's current state
+ * (between whole steps) and waits for them; the step entry points above are
+ * the product path.  After one that changes the mask (Onetime_init,
+ * Find_Calculation_limits, Reset_states) the next step re-derives its active
+ * tiles; call hd_prepare first when the segment tables should follow a mask
+ * changed otherwise. */
+/* Onetime_init, support_funcs.cu:20-58 (rep.py:126-130). */
+int hd_k_Onetime_init(hd_ctx* c);
+/* Find_Calculation_limits_Horizontal / _Vertical with the FS edge
+ * interpolations, support_funcs.cu:189-335 (rep.py:135-136). */
+int hd_k_Find_Calculation_limits(hd_ctx* c);
+/* Htuongdoi, support_funcs.cu:338-354. */
+int hd_k_Htuongdoi(hd_ctx* c);
+/* preprocess_data, support_funcs.cu:433-542 (rep.py:139). */
+int hd_k_preprocess_data(hd_ctx* c);
+/* Update_Boundary_Value at time t (support_funcs.cu:544-668), or with
+ * analytic != 0 the channel tide boundary_up/down/left/right
+ * (support_funcs.cu:356-429), Engine.py:29-46. */
+int hd_k_Update_Boundary_Value(hd_ctx* c, double t, int analytic);
+/* The UZ / VZ half-step solve of every line: _calculate_preindex, _abcd,
+ * _matrix_coeff, tridiagSolver and _extract_solution
+ * (UVZSolver_multithread.cu:203-723, 818-943; Engine.py:160-178, 269-296);
+ * shift 1 is the narrow-channel start index (Engine.py:152-158, 273-279). */
+int hd_k_UZSolver(hd_ctx* c, int shift);
+int hd_k_VZSolver(hd_ctx* c, int shift);
+/* The tridiagonal rows alone (UZ/VZSolver_calculate_matrix_coeff,
+ * UVZSolver_multithread.cu:469-593), written to the caller's arrays in the
+ * reference's layout: line L (column j for UZ, M+3 lines; row i for VZ, N+3
+ * lines), row r of the run starting at first
+ * at L*(2N+1) + 2*first + r for UZ, L*(2M+1) + 2*first + r for VZ; SN gets
+ * the system sizes, (max(N,M)+3) x segment_limit.  Positions outside the
+ * systems' rows hold unspecified values.  Any pointer may be NULL. */
+int hd_k_UZSolver_calculate_matrix_coeff(hd_ctx* c, int shift, double* AA, double* BB, double* CC, double* DD,
+                                         int* SN);
+int hd_k_VZSolver_calculate_matrix_coeff(hd_ctx* c, int shift, double* AA, double* BB, double* CC, double* DD,
+                                         int* SN);
+/* Normalize + update_buffer of t_u (isU) or t_v, support_funcs.cu:682-729. */
+int hd_k_Normalize(hd_ctx* c, int isU);
+/* solveV + update_margin_elem_V (UVZSolver_multithread.cu:727-768,
+ * 1008-1071), solveU + update_margin_elem_U (:773-815, 947-1006). */
+int hd_k_solveV(hd_ctx* c);
+int hd_k_solveU(hd_ctx* c);
+/* update_h_moi, Reset_states_horizontal (horizontal != 0) or _vertical,
+ * update_uvz with the narrow-channel flags and Htuongdoi, in that order
+ * (support_funcs.cu:60-186, 671-680, 338-354; Engine.py:199-214). */
+int hd_k_Reset_states(hd_ctx* c, int horizontal, int kenhhepd, int kenhhepng);
+/* Find_VTH and hesoK, Sediment_Transport.cu:508-568. */
+int hd_k_Find_VTH_hesoK(hd_ctx* c);
+/* Scan_FSj, tridiagSolver, FSj_extract_solution and Update_FS at time t
+ * (Sediment_Transport.cu:205-338, 594-629; Engine.py:218-239); FSi the
+ * same along rows (:62-202, 570-591; Engine.py:338-361). */
+int hd_k_FSj_solve(hd_ctx* c, double t, double s_start, int ketdinh);
+int hd_k_FSi_solve(hd_ctx* c, double t, double s_start, int ketdinh);
+/* The concentration rows alone (_FSj / _FSi_calculate__mactrix_coeff) in the
+ * reference's layout: FSj row of cell (i, j) at j*(N+3) + i, FSi at
+ * i*(M+3) + j; positions outside the systems' rows hold unspecified values. */
+int hd_k_Scan_FSj(hd_ctx* c, double t, double s_start, int ketdinh, double* AA, double* BB, double* CC,
+                  double* DD);
+int hd_k_Scan_FSi(hd_ctx* c, double t, double s_start, int ketdinh, double* AA, double* BB, double* CC,
+                  double* DD);
+/* Calculate_Qb (Sediment_Transport.cu:341-414) and BedLoad over rows
+ * [sidx, eidx) (:416-506, 630-639). */
+int hd_k_Calculate_Qb(hd_ctx* c, int qb_mode);
+int hd_k_BedLoad(hd_ctx* c, int ketdinh, int sidx, int eidx);
+
 #ifdef __cplusplus
 }
 #endif

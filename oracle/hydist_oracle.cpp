@@ -1736,13 +1736,16 @@ int or_set_threads(int n)
  *   bed_load (a = ketdinh, b = sidx, c = eidx),
  *   sediment_j / sediment_i (a = ketdinh, t; the scan, solve, extraction and
  *       update of the FS sweep, bed-change start taken from c as seconds),
- *   boundary (t: Update_Boundary_Value), channel_boundary (t). */
+ *   boundary (t: Update_Boundary_Value), channel_boundary (t),
+ *   onetime_init, preprocess. */
 int or_kernel(void* h, const char* name, int a, int b, int c, double t)
 {
     Oracle& o = *(Oracle*)h;
     o.err = 0;
     std::string n(name);
-    if (n == "uz_preindex") uz_preindex(o, a, b);
+    if (n == "onetime_init") onetime_init(o);
+    else if (n == "preprocess") preprocess(o);
+    else if (n == "uz_preindex") uz_preindex(o, a, b);
     else if (n == "vz_preindex") vz_preindex(o, a, b);
     else if (n == "uz_solve") uz_solve(o, a, b);
     else if (n == "vz_solve") vz_solve(o, a, b);
