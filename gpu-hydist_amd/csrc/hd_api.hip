@@ -12,6 +12,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1414,7 +1415,7 @@ int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const d
         hipMemcpy(dC, C, rb, hipMemcpyHostToDevice);
         hipMemcpy(dD, D, rb, hipMemcpyHostToDevice);
         hipMemcpy(dx, x, rb, hipMemcpyHostToDevice);
-        hd_thomas_batch_device(nsys, doff, dsn, dA, dB, dC, dD, dx, dAp, dBp, 0);
+        hd_thomas_batch_device(nsys, doff, dsn, dA, dB, dC, dD, dx, dAp, dBp, nrows, 0);
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = fail(nullptr, HD_EHIP, hipGetErrorString(e));
         else hipMemcpy(x, dx, rb, hipMemcpyDeviceToHost);
@@ -1631,6 +1632,48 @@ int hd_k_BedLoad(hd_ctx* c, int ketdinh, int sidx, int eidx)
     if (sidx < 2 || eidx > c->N || sidx > eidx) return fail(c, HD_EINVAL, "hd_k_BedLoad: rows out of range");
     hd_launch_bedload(x.b(), ketdinh, sidx, c->N - eidx, 0, x.stream);
     return unit_end(c, false);
+}
+
+/* ------------------------------------------------ batched Thomas, no allocation */
+
+/* Elimination-factor scratch of hd_thomas_batch_noalloc: Ap in the first
+ * HD_THOMAS_NOALLOC_MAX_ROWS doubles, Bp in the second (16 MiB per device,
+ * held until the process ends).  Host code never uses the symbol's name as
+ * an address: each device's copy is reached through the address that
+ * hipGetSymbolAddress returned on that device (g_scratch_ptr). */
+static __device__ __align__(256) double scratch[2 * (size_t)HD_THOMAS_NOALLOC_MAX_ROWS];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64];
+
+int hd_thomas_batch_noalloc(int device, int nsys, const int* off, const int* sn, const double* A, const double* B,
+                            const double* C, const double* D, double* x, int nrows, void* stream)
+{
+    if (nsys < 0 || nrows < 0 || (nsys > 0 && (!off || !sn || !A || !B || !C || !D || !x)))
+        return fail(nullptr, HD_EINVAL, "hd_thomas_batch_noalloc: bad arguments");
+    if (nrows > HD_THOMAS_NOALLOC_MAX_ROWS)
+        return fail(nullptr, HD_EINVAL, "hd_thomas_batch_noalloc: nrows " + std::to_string(nrows) +
+                                            " exceeds the scratch limit of " +
+                                            std::to_string(HD_THOMAS_NOALLOC_MAX_ROWS) + " rows");
+    if (device < 0 || device >= 64) return fail(nullptr, HD_EINVAL, "hd_thomas_batch_noalloc: bad device");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, HD_EHIP, "hipSetDevice");
+    double* base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fail(nullptr, HD_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            g_scratch_ptr[device] = (double*)p;
+        }
+        base = g_scratch_ptr[device];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hd_thomas_batch_device(nsys, off, sn, A, B, C, D, x, base, base + HD_THOMAS_NOALLOC_MAX_ROWS, nrows, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, HD_EHIP, std::string("hd_thomas_batch_noalloc: ") + hipGetErrorString(e));
+    return HD_OK;
 }
 
 } // extern "C"

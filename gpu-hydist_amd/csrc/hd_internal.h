@@ -156,7 +156,9 @@ void hd_launch_wet_count(const HdBatch& b, hipStream_t s);
 void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s);
 void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, hipStream_t s);
 void hd_launch_z_change(const HdBatch& b, const double* zref, double* out, hipStream_t s);
+/* batched Thomas on device arrays; systems outside rows [0, nrows) are skipped */
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
-                           const double* C, const double* D, double* x, double* Ap, double* Bp, hipStream_t s);
+                           const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,
+                           hipStream_t s);
 
 #endif

@@ -2857,13 +2857,16 @@ __global__ void k_wet_count(const HdDev* __restrict__ dv)
 
 /* ================================================================ batched Thomas */
 
+/* One system per lane.  A system outside rows [0, nrows) is skipped (its x
+ * is left as it is): the host checks the tables where it holds them, and this
+ * check covers tables that only the device holds. */
 __global__ void k_thomas_batch(int nsys, const int* off, const int* sn, const double* A, const double* B,
-                               const double* C, const double* D, double* x, double* Ap, double* Bp)
+                               const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     int o = off[s], n = sn[s];
-    if (n < 0) return;
+    if (n < 0 || o < 0 || n >= nrows - o) return;
     if (n == 0) { x[o] = D[o] / B[o]; return; }
     double ap = -C[o] / B[o], bp = D[o] / B[o];
     Ap[o] = ap; Bp[o] = bp;
@@ -3059,9 +3062,12 @@ void hd_launch_wet_count(const HdBatch& b, hipStream_t s)
 }
 
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
-                           const double* C, const double* D, double* x, double* Ap, double* Bp, hipStream_t s)
+                           const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,
+                           hipStream_t s)
 {
-    hipLaunchKernelGGL(k_thomas_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap, Bp);
+    if (nsys > 0)
+        hipLaunchKernelGGL(k_thomas_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap,
+                           Bp, nrows);
     return 0;
 }
 

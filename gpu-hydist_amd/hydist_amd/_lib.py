@@ -25,7 +25,7 @@ EXPORTS = (
     "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
     "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
-    "hd_z_snapshot", "hd_z_change",
+    "hd_z_snapshot", "hd_z_change", "hd_thomas_batch_noalloc",
     "hd_k_Onetime_init", "hd_k_Find_Calculation_limits", "hd_k_Htuongdoi", "hd_k_preprocess_data",
     "hd_k_Update_Boundary_Value", "hd_k_UZSolver", "hd_k_VZSolver", "hd_k_UZSolver_calculate_matrix_coeff",
     "hd_k_VZSolver_calculate_matrix_coeff", "hd_k_Normalize", "hd_k_solveV", "hd_k_solveU", "hd_k_Reset_states",
@@ -79,6 +79,7 @@ def load() -> ctypes.CDLL:
         "hd_profile": (ci, [vp, ci]),
         "hd_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
         "hd_thomas_batch": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
+        "hd_thomas_batch_noalloc": (ci, [ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, vp]),
         "hd_k_Onetime_init": (ci, [vp]),
         "hd_k_Find_Calculation_limits": (ci, [vp]),
         "hd_k_Htuongdoi": (ci, [vp]),

@@ -302,6 +302,22 @@ int hd_k_Scan_FSi(hd_ctx* c, double t, double s_start, int ketdinh, double* AA, 
 int hd_k_Calculate_Qb(hd_ctx* c, int qb_mode);
 int hd_k_BedLoad(hd_ctx* c, int ketdinh, int sidx, int eidx);
 
+/* hd_thomas_batch without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * pointer is DEVICE memory the caller owns (off, sn: nsys ints; A, B, C, D,
+ * x: nrows doubles); the work is enqueued on `stream` (a hipStream_t, or
+ * NULL for the null stream) and the call returns without waiting.  The
+ * elimination factors live in a static device array of the library, so
+ * nrows may be at most HD_THOMAS_NOALLOC_MAX_ROWS (HD_EINVAL beyond it).
+ * The tables stay on the device, so a system outside rows [0, nrows) is
+ * skipped rather than reported.  Calls that overlap on one device share that
+ * array and race: serialise them (one stream, or events). */
+#define HD_THOMAS_NOALLOC_MAX_ROWS (1 << 20)
+int hd_thomas_batch_noalloc(int device, int nsys, const int* off, const int* sn, const double* A,
+                            const double* B, const double* C, const double* D, double* x, int nrows,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif
