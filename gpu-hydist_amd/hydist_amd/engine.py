@@ -13,7 +13,10 @@ a replayed HIP graph per step, no host synchronisation inside).
 New keyword options: ``boundary_mode`` ('auto' = the reference rule, channel
 flags force the analytic tide, Engine.py:29-46; 'series'; 'analytic'),
 ``qb_mode`` ('fixed' or 'reference', Sediment_Transport.cu:365) and
-``bed_mode`` ('periodic', Engine.py:364, or 'every_step').
+``bed_mode`` ('periodic', Engine.py:364, or 'every_step').  With ``debug``
+the half-steps run one by one so that t_u, t_v, t_z are recorded after each,
+as the reference does; where the reference calls exit() on a NaN in debug
+mode (Engine.py:383-393) this returns the lists gathered so far.
 """
 from __future__ import annotations
 
@@ -43,6 +46,16 @@ def step_options(kenhhepd=0, kenhhepng=0, ketdinh=True, sediment_start=36000, be
     o.sediment_start = float(sediment_start)
     o.bed_change_start = float(bed_change_start)
     return o
+
+
+def _bed_trigger(t, o: HdStepOpts) -> bool:
+    """Whether the step ending at t changes the bed (Engine.py:364; every
+    step from bed_change_start in the every_step mode)."""
+    if o.bed_mode == BED_EVERY_STEP:
+        return t >= o.bed_change_start
+    if t != int(t) or o.bed_change_start == 0:
+        return False
+    return math.fmod(int(t), o.bed_change_start) == 0
 
 
 def _event(t, interval, sediment_start, save_grid, export, debug):
@@ -78,22 +91,30 @@ def hydraulic_Calculation(Tmax, pointers, arg_struct_ptr, arr_struct_ptr, supmod
     FS = np.zeros(shape)
     t = float(t_start)
     while t < Tmax:
-        # steps until the next host event (or Tmax): replayed without syncs
-        n, tt = 0, t
-        while tt < Tmax:
-            tt = tt + dT * 0.5
-            tt = tt + dT * 0.5
-            n += 1
-            if _event(tt, interval, sediment_start, save_grid, export, debug):
-                break
-        t = solver.run(t, n, opts)
         if debug:
-            pointers.extract({"u": gpu_u, "v": gpu_v, "z": gpu_z})
-            u_list.append(solver.get("t_u").copy())
-            v_list.append(solver.get("t_v").copy())
-            z_list.append(solver.get("t_z").copy())
-            if np.isnan(gpu_u).any() or np.isnan(gpu_v).any() or np.isnan(gpu_z).any():
-                return u_list, v_list, z_list, fslist
+            # the reference records t_u, t_v, t_z after each half-step
+            # (Engine.py:245-257, 369-393) and stops on a NaN in u, v or z
+            for which in (0, 1):
+                t = t + dT * 0.5
+                solver.halfstep(which, t, opts)
+                if which == 1 and _bed_trigger(t, opts):
+                    solver.bed_change(t, opts)
+                pointers.extract({"u": gpu_u, "v": gpu_v, "z": gpu_z})
+                u_list.append(solver.get("t_u").copy())
+                v_list.append(solver.get("t_v").copy())
+                z_list.append(solver.get("t_z").copy())
+                if np.isnan(gpu_u).any() or np.isnan(gpu_v).any() or np.isnan(gpu_z).any():
+                    return u_list, v_list, z_list, fslist
+        else:
+            # steps until the next host event (or Tmax): replayed without syncs
+            n, tt = 0, t
+            while tt < Tmax:
+                tt = tt + dT * 0.5
+                tt = tt + dT * 0.5
+                n += 1
+                if _event(tt, interval, sediment_start, save_grid, export, debug):
+                    break
+            t = solver.run(t, n, opts)
         integral = t - int(t) == 0
         if interval and integral and int(t) % interval == 0:
             pointers.extract({"u": gpu_u, "v": gpu_v, "z": gpu_z, "FS": FS})

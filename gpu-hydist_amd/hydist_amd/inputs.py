@@ -11,6 +11,7 @@ non-solid edge's series divided by its number of typed segments.
 from __future__ import annotations
 
 import os
+import warnings
 from typing import Dict, Optional
 
 import numpy as np
@@ -38,8 +39,24 @@ def _map(path: str, nangday: float = 0.0) -> np.ndarray:
     return np.pad(a, ((1, 2), (1, 2)), "edge") + nangday
 
 
+def device_view(a: np.ndarray, shape) -> np.ndarray:
+    """What a kernel reads from a map uploaded at its own size and indexed
+    with the grid's row width (Pointers.py:29-33 allocates each array at its
+    nbytes; the kernels read flat index i*(M+3)+j): the first
+    (N+3)*(M+3) values of the row-major map, reshaped to the grid."""
+    flat = np.ascontiguousarray(a).reshape(-1)
+    n = shape[0] * shape[1]
+    if flat.size < n:
+        raise ValueError(f"map of {flat.size} values is smaller than the {shape[0]}x{shape[1]} grid")
+    return flat[:n].reshape(shape).copy()
+
+
 def load_case(dirs_path: str, header_text: str, segment_limit: Optional[int] = None, name: str = "case",
-              hours: float = 1.0) -> Case:
+              hours: float = 1.0, mismatched_maps: str = "reference") -> Case:
+    """The case a DirsOfInputs file describes.  A map whose shape differs
+    from the topography's (TanChau's viscosity map is 335 x 465 against
+    324 x 464) is, with mismatched_maps="reference", taken as the reference's
+    device reads it (device_view, with a warning); "error" raises instead."""
     pdict = parse_case_header(header_text)
     SL = int(segment_limit or pdict["segment_limit"])
     nangday = float(pdict["NANGDAY"])
@@ -64,12 +81,17 @@ def load_case(dirs_path: str, header_text: str, segment_limit: Optional[int] = N
     N, M = h.shape[0] - 3, h.shape[1] - 3
     a = empty_state(N, M, SL)
     a["h"] = h
-    a["hsnham"] = _map(D["roughness_map"])
-    visc = _map(D["visco_map"])
-    if visc.shape != h.shape:
-        raise ValueError(f"viscosity map has shape {visc.shape[0]-3}x{visc.shape[1]-3}, topography {N}x{M}")
-    a["VISCOINDX"] = visc
-    a["Fw"] = _map(D["friction_map"])
+    for key, field in (("roughness_map", "hsnham"), ("visco_map", "VISCOINDX"), ("friction_map", "Fw")):
+        m = _map(D[key])
+        if m.shape != h.shape:
+            msg = (f"{key} has shape {m.shape[0] - 3}x{m.shape[1] - 3} after the transpose, the topography "
+                   f"{N}x{M}")
+            if mismatched_maps != "reference":
+                raise ValueError(msg)
+            warnings.warn(msg + "; taking the values the reference's kernels read (its first "
+                                f"{(N + 3) * (M + 3)} values, row-major)")
+            m = device_view(m, h.shape)
+        a[field] = m
     a["boundary_type"] = bt
     bienQ = np.zeros(4, np.int32)
     bienQ[np.where(bt == 2)[0]] = 1

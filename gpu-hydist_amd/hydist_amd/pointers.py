@@ -109,10 +109,26 @@ class Pointers:
             if key in self.arg_list:
                 value[...] = self._solver().get(self.arg_list[key].name, like=value).reshape(value.shape)
 
-    def debug(self, keyMap):
-        raise NotImplementedError(
-            "device-only Thomas scratch (Pointers.debug) is not materialised: the coefficients "
-            "live in registers and the line-major row buffers of the current half-step")
+    def debug(self, keyMap, direction="VZ"):
+        """Pointers.py:158-162 copies the device-only Thomas scratch.  Here the
+        coefficients live in registers and LDS during a sweep, so the rows are
+        assembled on request from the current state by the library's row
+        kernel (hd_k_UZ/VZSolver_calculate_matrix_coeff): keys AA, BB, CC, DD
+        and SN, in the reference's layout (line * (2M+1) + row for VZ, the
+        direction of the last assembly of a step without sediment).  The
+        reference's buffers instead hold the rows of the last half-step's
+        starting state.  Other scratch names (x, Ap, Bp, ep, a1..f5) have no
+        device counterpart and raise KeyError."""
+        s = self._solver()
+        name = ("UZ" if direction == "UZ" else "VZ") + "Solver_calculate_matrix_coeff"
+        AA, BB, CC, DD, SN = s.rows(name, 0)
+        got = {"AA": AA, "BB": BB, "CC": CC, "DD": DD, "SN": SN}
+        for key, des in keyMap.items():
+            if key not in got:
+                raise KeyError(f"{key}: no device counterpart of this scratch array")
+            src = got[key]
+            n = min(des.size, src.size)
+            des.reshape(-1)[:n] = src[:n]
 
 
 class PointersStruct:

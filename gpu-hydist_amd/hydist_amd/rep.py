@@ -89,6 +89,22 @@ def load_intial_condition(dirs, pointers):
     pointers.toDevice(["u", "v", "z", "khouot", "FS"])
 
 
+def save_pick_up(dirs, pointers, t=0.0):
+    """The files load_intial_condition reads (rep.py:43-55), which nothing in
+    the reference writes: u, v, z with one header line, khouot and FS, each
+    the (N+2) x (M+2) leading block of the padded field, written with 17
+    significant digits so that a restart reproduces the doubles exactly."""
+    os.makedirs(dirs, exist_ok=True)
+    s = pointers._solver()
+    N, M = s.N, s.M
+    for name in ("u", "v", "z"):
+        a = s.get(name)[:N + 2, :M + 2]
+        np.savetxt(os.path.join(dirs, name + ".txt"), a, fmt="%.17g", header=f"{name} N={N} M={M} t={t!r}",
+                   comments="")
+    np.savetxt(os.path.join(dirs, "khouot.txt"), s.get("khouot")[:N + 2, :M + 2], fmt="%d")
+    np.savetxt(os.path.join(dirs, "FS.txt"), s.get("FS")[:N + 2, :M + 2], fmt="%.17g")
+
+
 def gpu_init(device_no, pick_up, pickup_dirs, case):
     ctx = Context(device_no, case)
     a = case.arrays
