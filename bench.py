@@ -4,15 +4,22 @@ MI355X, in wet-cell-steps per second.
 
 A step is one full ADI step (UZ + VZ half-steps) of the whole river; the
 timed region replays K captured steps (hd_run) with the state resident in
-HBM.  One JSON line is printed with the throughput, the roofline of the
-dominant kernel (the kernel function with the largest total time per step,
-its UZ and VZ launches together; per-launch HIP-event durations recorded on the library's
-stream inside this process; HBM traffic per launch from the rocprofv3
-counter runs recorded in profiles/traffic.json), the CPU checker timed on a
-bounded sample on this host's cores, and the batched-ensemble workload (64
+HBM, after W warm-up steps.  A clone of the state at the start of the timed
+window then runs the same K steps again with HIP events around every launch
+on the library's stream, so the roofline of the dominant kernel (the kernel
+function with the largest total time per step, its UZ and VZ launches
+together) is measured on exactly the timed steps; its HBM traffic per launch
+comes from the rocprofv3 counter runs recorded in profiles/traffic.json.
+One JSON line is printed, with the CPU checker timed on this host (median of
+three runs, -O3 -march=native built here) and the batched ensemble (64
 seeded rivers, one launch per kernel over all of them).
 
     python bench.py [--gpus 1] [--steps 720] [--warmup 20] [--no-cpu] [--no-ensemble]
+
+--workload sediment | ensemble | cnn prints the line of another subsystem
+instead (the river with suspended sediment and bed change every step; the
+ensemble alone, with per-kernel times; the surrogate network's forward pass
+and training step), for profiling each subsystem with rocprofv3.
 """
 from __future__ import annotations
 
@@ -75,7 +82,27 @@ KERNEL_BYTES = {
     "adi_solve_vz": 2 * 4 * 8 + 2 * 8,
     "adi_coef_uz": 9 * 8 + 2 * 4 * 8,
     "adi_coef_vz": 9 * 8 + 2 * 4 * 8,
+    # suspended sediment: Find_VTH + hesoK (t_u, t_v, H_moi, htaiz, Htdu,
+    # Htdv, h read; VTH, Kx, Ky written)
+    "vth_hesok": 7 * 8 + 3 * 8,
+    # the concentration sweeps (FS, H_moi, t_u, t_v, VTH, Htdu, Htdv, Fw, Kx,
+    # Ky read; tFS written)
+    "fs_spec_j": 10 * 8 + 8,
+    "fs_spec_i": 10 * 8 + 8,
+    # Update_FS: tFS read, FS written
+    "update_fs": 2 * 8,
+    # bed-load flux: Fw, VTH, H_moi, t_u, t_v and khouot read; Qbx, Qby written
+    "calc_qb": 5 * 8 + 4 + 2 * 8,
+    # bed change: FS, H_moi, t_u, t_v, VTH, Htdu, Htdv, Fw, Kx, Ky, Qbx, Qby,
+    # dH and khouot read; dH written
+    "bedload": 13 * 8 + 4 + 8,
 }
+
+# Surrogate network: multiply-adds per pixel of one forward pass (3 x 3
+# convolutions 8 -> 64, six 64 -> 64, 64 -> 3; two flops each); a training
+# step (forward, data and weight gradients) is three times that.
+CNN_FLOP_FWD = 2 * 9 * (8 * 64 + 6 * 64 * 64 + 64 * 3)
+MFMA_BF16_PEAK = 2.5e15
 
 
 def parse():
@@ -84,10 +111,16 @@ def parse():
     p.add_argument("--steps", type=int, default=720)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--profile-steps", type=int, default=20)
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-steps", type=int, default=400, help="steps per CPU run (all host threads)")
+    p.add_argument("--cpu-steps-single", type=int, default=200, help="steps per single-thread CPU run")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--ensemble-steps", type=int, default=720)
     p.add_argument("--no-ensemble", action="store_true")
+    p.add_argument("--workload", default="river", choices=("river", "sediment", "ensemble", "cnn"),
+                   help="river: the headline line (default); the others print their own line "
+                        "(sediment: the river with suspended sediment and bed change every step; "
+                        "ensemble: the 64-river batch; cnn: the surrogate network's passes)")
+    p.add_argument("--cnn-reps", type=int, default=20)
     return p.parse_args()
 
 
@@ -149,31 +182,41 @@ def roofline(kt, wet_mean, traffic_file=None):
     return roof
 
 
-def cpu_baseline(case, budget_s, threads):
-    """The CPU checker (same FP64 method, OpenMP over independent lines) on
-    the steps of the same run from t = 0 that fit in about `budget_s`
-    seconds; returns (wet-cell-steps/s, steps, seconds)."""
+def cpu_run(case, steps):
+    """One timed run of the CPU checker: `steps` steps of the workload from
+    t = 0 (initialisation excluded); returns (wet-cell-steps/s, seconds)."""
     import oracle
-    used = oracle.set_threads(threads)
     o = oracle.Oracle(case)
     o.init()
     opts = case.opts()
     wet = 0
     t = 0.0
-    n = 0
     t0 = time.perf_counter()
-    while True:
+    for _ in range(steps):
         wet += o.wet_count()
         t = o.step(t, opts)
-        n += 1
-        dt = time.perf_counter() - t0
-        if dt >= budget_s or n >= 100000:
-            break
+    dt = time.perf_counter() - t0
     o.close()
-    return wet / dt, n, dt, used
+    return wet / dt, dt
 
 
-def ensemble_bench(nsteps):
+def cpu_baseline(case, steps, threads, runs=3):
+    """The CPU checker (same FP64 method, OpenMP over independent lines),
+    median of `runs` runs of `steps` steps each on `threads` threads; returns
+    (median wet-cell-steps/s, seconds per run, threads used)."""
+    import oracle
+    used = oracle.set_threads(threads)
+    out = [cpu_run(case, steps) for _ in range(runs)]
+    rates = sorted(r for r, _ in out)
+    return rates[len(rates) // 2], [round(d, 2) for _, d in out], used
+
+
+def cpu_host():
+    """(logical CPUs of this host, model name)."""
+    return os.cpu_count(), cpu_model()
+
+
+def ensemble_bench(nsteps, profile=False):
     """The batched-ensemble workload: 64 seeded meandering rivers (sides
     256-1024, own discharge, roughness, tide; sediment on), one launch per
     kernel over all of them, nsteps steps (720 = one simulated hour)."""
@@ -220,66 +263,97 @@ def ensemble_bench(nsteps):
     cells = sum((c.N + 3) * (c.M + 3) for c in cases)
     rate = wet_steps / el
     redrawn = sum(1 for c in cases if not c.name.split("_")[0].endswith(".0"))
+    kt = None
+    if profile:       # per-kernel launch durations over 20 more steps
+        ens.profile(True)
+        ens.run(t, 20, op)
+        kt = ens.kernel_times()
+        ens.profile(False)
     out = {"workload": f"ensemble_{len(cases)}_rivers_256-1024", "scenarios": len(cases), "padded_cells": cells,
            "redrawn_members": redrawn,
            "steps": nsteps, "sim_hours": nsteps * cases[0].dT / 3600.0, "value": rate,
            "unit": "wet-cell-steps/s", "ms_per_step": el * 1e3 / nsteps,
            "mean_wet_cells": wet_steps / nsteps, "members_finite": finite,
            "hbm_frac_step_algorithmic": rate * B_STEP_SED / HBM_PEAK, "setup_s": round(setup, 1)}
+    if profile:
+        out["kt"] = kt
     ens.close()
     for m in ms:
         m.close()
     return out
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
-        sys.exit(2)
-    from hydist_amd import synth
+def timed_window(case, opts, warmup, steps):
+    """Warm-up steps, then `steps` steps replayed from captured HIP graphs
+    with the state in HBM, timed; a clone of the state taken at the start of
+    the timed window then runs the same steps again with HIP events around
+    every launch (on the library's stream), so the per-kernel durations
+    belong to exactly the timed steps.  Returns (timing dict, solver)."""
     from hydist_amd._lib import Solver
-
-    case = synth.config(2)
-    opts = case.opts()
     g = Solver(case, device=0)
     g.init()
-    t = g.run(0.0, a.warmup, opts) if a.warmup else 0.0
+    t = g.run(0.0, warmup, opts) if warmup else 0.0
     g.sync()
+    shadow = g.clone()
+    t_start = t
     wc0 = g.wet_count()
     g.reset_wet_counter()
     g.sync()
     t0 = time.perf_counter()
-    t = g.run(t, a.steps, opts, counted=True)
+    t = g.run(t, steps, opts, counted=True)
     g.sync()
     elapsed = time.perf_counter() - t0
-    acc = g.wet_cell_steps()                     # end states of the K timed steps
+    acc = g.wet_cell_steps()                     # end states of the timed steps
     wc1 = g.wet_count()
-    wet_steps = acc + wc0 - wc1                  # start states of the K timed steps
-    value = wet_steps / elapsed
-    ms_per_step = elapsed * 1e3 / a.steps
+    wet_steps = acc + wc0 - wc1                  # start states of the timed steps
+    shadow.profile(True)
+    shadow.run(t_start, steps, opts)
+    kt = shadow.kernel_times()
+    shadow.profile(False)
+    shadow.close()
+    return {"value": wet_steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "wet_mean": wet_steps / steps,
+            "kt": kt, "t": t}, g
 
-    # per-kernel launch durations (HIP events on the library's stream)
-    g.profile(True)
-    g.run(t, a.profile_steps, opts)
-    kt = g.kernel_times()
-    g.profile(False)
-    wet_mean = wet_steps / a.steps
-    roof = roofline(kt, wet_mean)
-    kernels = {k: round(ms / n * 1e3, 2) for k, (ms, n) in kt.items()}
+
+def kernel_table(kt, wet_mean):
+    """Per kernel label: average launch (us) and achieved algorithmic GB/s."""
+    out = {}
+    for k, (ms, n) in kt.items():
+        us = ms / n * 1e3
+        row = {"us": round(us, 2), "launches": n}
+        if k in KERNEL_BYTES:
+            row["GB/s"] = round(KERNEL_BYTES[k] * wet_mean / (us * 1e-6) / 1e9, 1)
+        out[k] = row
+    return out
+
+
+def river_line(a):
+    from hydist_amd import synth
+    case = synth.config(2)
+    opts = case.opts()
+    w, g = timed_window(case, opts, a.warmup, a.steps)
+    g.close()
+    roof = roofline(w["kt"], w["wet_mean"])
+    kernels = {k: round(ms / n * 1e3, 2) for k, (ms, n) in w["kt"].items()}
 
     cpu = None
     if not a.no_cpu:
-        rate, n, dt, used = cpu_baseline(case, a.cpu_seconds, cpu_threads())
+        import oracle
+        try:
+            lib_used = os.path.basename(oracle.use_native_build())
+            build = "-O3 -march=native built on this host"
+        except Exception as e:     # no compiler on the host: the prebuilt checker
+            lib_used, build = "liboracle.so", f"-O2 prebuilt (native build failed: {type(e).__name__})"
+        ncpu, model = cpu_host()
+        rate, secs, used = cpu_baseline(case, a.cpu_steps, cpu_threads())
         cpu = {"value": rate, "unit": "wet-cell-steps/s", "cores": used, "kind": "port",
-               "sample": f"first {n} steps of {case.name} from t=0 ({dt:.1f} s), CPU checker "
-                         f"(FP64 restatement, OpenMP over lines, {used} threads of {os.cpu_count()} "
-                         f"logical CPUs, {cpu_model()})"}
-        # the same checker on one thread, over half the time budget
-        rate1, n1, dt1, _ = cpu_baseline(case, a.cpu_seconds / 2, 1)
+               "sample": f"median of 3 runs of the first {a.cpu_steps} steps of {case.name} from t=0 "
+                         f"({secs} s), CPU checker (FP64 restatement, OpenMP over lines, {used} threads; "
+                         f"nproc {ncpu}, {model}; {lib_used}, {build})"}
+        rate1, secs1, _ = cpu_baseline(case, a.cpu_steps_single, 1)
         cpu["single_thread"] = {"value": rate1, "cores": 1,
-                                "sample": f"first {n1} steps of {case.name} from t=0 ({dt1:.1f} s)"}
+                                "sample": f"median of 3 runs of the first {a.cpu_steps_single} steps of "
+                                          f"{case.name} from t=0 ({secs1} s), one thread"}
 
     ens = None
     if not a.no_ensemble:
@@ -288,27 +362,104 @@ def main():
         except Exception as e:  # reported, the river line still prints
             ens = {"error": f"{type(e).__name__}: {e}"}
 
-    line = {
+    return {
         "metric": "ADI wet-cell-steps/s (FP64)",
-        "value": value,
+        "value": w["value"],
         "unit": "wet-cell-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": w["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": case.name, "N": case.N, "M": case.M, "dx_m": 20.0, "dT_s": case.dT,
-                   "mean_wet_cells": round(wet_mean, 1), "physics": "hydrodynamics",
-                   "hbm_frac_step_algorithmic": value * B_STEP / HBM_PEAK},
+                   "mean_wet_cells": round(w["wet_mean"], 1), "physics": "hydrodynamics",
+                   "hbm_frac_step_algorithmic": w["value"] * B_STEP / HBM_PEAK},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_us": kernels,
         "ensemble": ens,
     }
+
+
+def sediment_line(a):
+    """The river with suspended sediment and bed change every step (the
+    config-3 physics from t = 0)."""
+    from hydist_amd import synth
+    from hydist_amd.case import BED_EVERY_STEP
+    case = synth.config(3)
+    case.sediment_start = 0.0
+    case.bed_change_start = 0.0
+    case.bed_mode = BED_EVERY_STEP
+    opts = case.opts()
+    w, g = timed_window(case, opts, a.warmup, a.steps)
+    g.close()
+    return {"metric": "ADI wet-cell-steps/s with sediment and bed change (FP64)", "value": w["value"],
+            "unit": "wet-cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": w["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": case.name, "physics": "hydrodynamics + suspended sediment + bed change",
+                       "mean_wet_cells": round(w["wet_mean"], 1),
+                       "hbm_frac_step_algorithmic": w["value"] * (B_STEP_SED + 16) / HBM_PEAK},
+            "roofline": roofline(w["kt"], w["wet_mean"]), "kernels": kernel_table(w["kt"], w["wet_mean"])}
+
+
+def ensemble_line(a):
+    e = ensemble_bench(a.ensemble_steps, profile=True)
+    kt = e.pop("kt")
+    return {"metric": "ensemble wet-cell-steps/s (FP64)", **e, "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "roofline": roofline(kt, e["mean_wet_cells"]), "kernels": kernel_table(kt, e["mean_wet_cells"])}
+
+
+def cnn_line(a):
+    """The surrogate network on the river's padded grid (batch 1): forward
+    with the loss (hd_cnn_eval_samples) and a training step (forward, data
+    and weight gradients, Adam: hd_cnn_train_samples) on a device-resident
+    sample captured from the solver."""
+    from hydist_amd import cnn, synth
+    from hydist_amd._lib import Solver
+    case = synth.config(2)
+    g = Solver(case, device=0)
+    g.init()
+    g.run(0.0, 10, case.opts())
+    H, W = case.N + 3, case.M + 3
+    net = cnn.Network(H, W, batch=1)
+    net.data_alloc(1)
+    net.capture(0, 0, g)
+    net.capture(0, 1, g)
+    out = {}
+    for name, fn, mult in (("forward", lambda: net.eval_samples([0]), 1),
+                           ("train_step", lambda: net.train_samples([0], 1e-6), 3)):
+        fn()
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(a.cnn_reps):
+            fn()
+        el = (time.perf_counter() - t0) / a.cnn_reps
+        flop = CNN_FLOP_FWD * mult * H * W
+        out[name] = {"ms": el * 1e3, "TFLOP/s": flop / el / 1e12, "frac_bf16_dense_peak": flop / el / MFMA_BF16_PEAK}
+    net.close()
+    g.close()
+    f = out["forward"]
+    return {"metric": "surrogate network TFLOP/s (BF16 MFMA, FP32 accumulate)", "value": f["TFLOP/s"],
+            "unit": "TFLOP/s", "n_gpus": 1, "higher_is_better": True, "dtype": "bf16", "data": "synthetic",
+            "config": {"workload": f"cnn_8layer_w64_{H}x{W}", "batch": 1, "reps": a.cnn_reps,
+                       "flop_per_pixel_forward": CNN_FLOP_FWD},
+            "roofline": {"bound": "mfma", "achieved": f["TFLOP/s"], "peak": MFMA_BF16_PEAK / 1e12,
+                         "unit": "TFLOP/s", "frac": f["frac_bf16_dense_peak"], "traffic": None},
+            "passes": out}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    line = {"river": river_line, "sediment": sediment_line, "ensemble": ensemble_line, "cnn": cnn_line}[a.workload](a)
     print(json.dumps(line))
 
 

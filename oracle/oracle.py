@@ -20,7 +20,21 @@ if _PKG not in sys.path:
 from hydist_amd.case import HdParams, HdStepOpts  # noqa: E402
 
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+NATIVE_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
 _lib = None
+_path = LIB_PATH
+
+
+def use_native_build() -> str:
+    """Build (make native: -O3 -march=native for this host) and select the
+    host-tuned checker for the CPU baseline; must precede the first use of
+    the checker in the process.  Returns the library path in use."""
+    global _path
+    if _lib is not None:
+        raise RuntimeError("the checker library is already loaded")
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "native"], check=True)
+    _path = NATIVE_PATH
+    return _path
 
 
 def build(force: bool = False) -> str:
@@ -33,8 +47,9 @@ def build(force: bool = False) -> str:
 def lib():
     global _lib
     if _lib is None:
-        build()
-        L = ctypes.CDLL(LIB_PATH)
+        if _path == LIB_PATH:
+            build()
+        L = ctypes.CDLL(_path)
         vp, cp, sz = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t
         L.or_create.restype = vp
         L.or_create.argtypes = [ctypes.POINTER(HdParams), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
