@@ -117,7 +117,7 @@ struct hd_ctx {
     int wet_mode = 0;
     int run_blocks_u = 0, run_blocks_v = 0;
     double* bed_snap = nullptr; /* dH snapshot for hd_bed_extrapolate (allocated on first use) */
-    double* z_snap = nullptr;   /* z snapshot for hd_z_change (allocated on first use), + 3 result slots */
+    double* z_snap = nullptr;   /* z snapshot for hd_z_change (allocated on first use), + 5 result slots */
     int ntiles = 0;             /* active row tiles (computed at init / prepare) */
     bool settled = false;       /* a full step has run since init / upload: land tiles are at a fixed point */
     bool sed_settled = false;   /* ... and a sediment sweep over the whole grid (VTH, hesoK, FS = tFS) */
@@ -227,7 +227,11 @@ static void derive_params(hd_params& p)
 }
 
 /* device allocation of a context, zero-filled; `state` allocations hold
- * data (copied by hd_clone), the others hold pointer tables */
+ * data (copied by hd_clone), the others hold pointer tables.  The fill runs
+ * on the context's own stream and has finished when this returns: the
+ * stream is non-blocking, so a fill on the null stream would not be ordered
+ * before the copies and kernels queued on it next (a clone's state copy, a
+ * snapshot) and could land after them. */
 static int dalloc(hd_ctx* c, void** p, size_t bytes, bool state = true)
 {
     if (bytes == 0) bytes = 8;
@@ -235,8 +239,9 @@ static int dalloc(hd_ctx* c, void** p, size_t bytes, bool state = true)
     if (e != hipSuccess) return fail(c, HD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->allocs.push_back(*p);
     if (state) c->state_allocs.push_back(std::make_pair(*p, bytes));
-    e = hipMemset(*p, 0, bytes);
-    if (e != hipSuccess) return fail(c, HD_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, HD_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
     return HD_OK;
 }
 
@@ -948,32 +953,40 @@ int hd_z_snapshot(hd_ctx* c)
     HIPCHK(c, hipSetDevice(c->device));
     size_t F = (size_t)(c->N + 3) * (c->M + 3);
     if (!c->z_snap) {
-        int rc = dalloc(c, (void**)&c->z_snap, (F + 3) * 8, false);
+        int rc = dalloc(c, (void**)&c->z_snap, (F + 5) * 8, false);
         if (rc) return rc;
     }
     HIPCHK(c, hipMemcpyAsync(c->z_snap, c->d.z, F * 8, hipMemcpyDeviceToDevice, c->stream));
     return HD_OK;
 }
 
-int hd_z_change(hd_ctx* c, double* max_abs_dz, double* wet_volume, double* wet_volume_snap)
+int hd_z_check(hd_ctx* c, double bound, double* max_abs_dz, double* frac_over, double* wet_volume,
+               double* wet_volume_snap)
 {
     if (!c) return HD_EINVAL;
-    if (!c->z_snap) return fail(c, HD_ESTATE, "hd_z_change before hd_z_snapshot");
+    if (!c->z_snap) return fail(c, HD_ESTATE, "hd_z_check before hd_z_snapshot");
+    if (!(bound >= 0)) return fail(c, HD_EINVAL, "hd_z_check: bound must be >= 0");
     HIPCHK(c, hipSetDevice(c->device));
     size_t F = (size_t)(c->N + 3) * (c->M + 3);
     double* res = c->z_snap + F;
-    double zero[3] = {0, 0, 0};
+    double zero[5] = {0, 0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(res, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
-    hd_launch_z_change(exec_of(c).b(), c->z_snap, res, c->stream);
+    hd_launch_z_change(exec_of(c).b(), c->z_snap, bound, res, c->stream);
     HIPCHK(c, hipGetLastError());
-    double out[3];
+    double out[5];
     HIPCHK(c, hipMemcpyAsync(out, res, sizeof(out), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     /* cell volumes are per unit area: the sums of depths */
     if (max_abs_dz) *max_abs_dz = out[0];
+    if (frac_over) *frac_over = out[4] > 0 ? out[3] / out[4] : 0.0;
     if (wet_volume) *wet_volume = out[1];
     if (wet_volume_snap) *wet_volume_snap = out[2];
     return HD_OK;
+}
+
+int hd_z_change(hd_ctx* c, double* max_abs_dz, double* wet_volume, double* wet_volume_snap)
+{
+    return hd_z_check(c, HUGE_VAL, max_abs_dz, nullptr, wet_volume, wet_volume_snap);
 }
 
 int hd_flow_refresh(hd_ctx* c)

@@ -428,11 +428,16 @@ static int cfail(hd_cnn* n, int code, const std::string& m)
         if (e_ != hipSuccess) return cfail((n), HD_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
     } while (0)
 
+/* zero-filled device allocation; the fill runs on the network's
+ * (non-blocking) stream and has finished when this returns, so it is
+ * ordered before everything queued on that stream afterwards */
 static int calloc_dev(hd_cnn* n, void** p, size_t bytes)
 {
     if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) return cfail(n, HD_ENOMEM, "hipMalloc (network)");
     n->allocs.push_back(*p);
-    if (hipMemset(*p, 0, bytes ? bytes : 8) != hipSuccess) return cfail(n, HD_EHIP, "hipMemset");
+    if (hipMemsetAsync(*p, 0, bytes ? bytes : 8, n->stream) != hipSuccess ||
+        hipStreamSynchronize(n->stream) != hipSuccess)
+        return cfail(n, HD_EHIP, "hipMemsetAsync");
     return HD_OK;
 }
 
@@ -655,10 +660,10 @@ int hd_cnn_set_layer(hd_cnn* n, int l, const float* host_w, const float* host_b)
     CHK(n, hipStreamSynchronize(n->stream));
     if (host_w) CHK(n, hipMemcpy(y.w32, host_w, y.nw() * 4, hipMemcpyHostToDevice));
     if (host_b) CHK(n, hipMemcpy(y.b32, host_b, (size_t)y.cout * 4, hipMemcpyHostToDevice));
-    CHK(n, hipMemset(y.mw, 0, y.nw() * 4));
-    CHK(n, hipMemset(y.vw, 0, y.nw() * 4));
-    CHK(n, hipMemset(y.mb, 0, (size_t)y.cout * 4));
-    CHK(n, hipMemset(y.vb, 0, (size_t)y.cout * 4));
+    CHK(n, hipMemsetAsync(y.mw, 0, y.nw() * 4, n->stream));
+    CHK(n, hipMemsetAsync(y.vw, 0, y.nw() * 4, n->stream));
+    CHK(n, hipMemsetAsync(y.mb, 0, (size_t)y.cout * 4, n->stream));
+    CHK(n, hipMemsetAsync(y.vb, 0, (size_t)y.cout * 4, n->stream));
     int rc = refresh_layer(n, l);
     if (rc) return rc;
     CHK(n, hipStreamSynchronize(n->stream));

@@ -155,7 +155,7 @@ void hd_launch_bedload(const HdBatch& b, int ketdinh, int slo, int eoff, int nti
 void hd_launch_wet_count(const HdBatch& b, hipStream_t s);
 void hd_launch_flow_refresh(const HdBatch& b, hipStream_t s);
 void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, hipStream_t s);
-void hd_launch_z_change(const HdBatch& b, const double* zref, double* out, hipStream_t s);
+void hd_launch_z_change(const HdBatch& b, const double* zref, double bound, double* out, hipStream_t s);
 /* batched Thomas on device arrays; systems outside rows [0, nrows) are skipped */
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
                            const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,

@@ -2333,14 +2333,17 @@ __global__ void __launch_bounds__(256) k_bed_extrapolate(const HdDev* __restrict
 /* Over the wet cells (khouot == 0) of [1,N]x[1,M]: max |z - zref| into
  * out[0] (as the bit pattern of a non-negative double, compared as an
  * unsigned integer), sum of htaiz + z into out[1], sum of htaiz + zref into
- * out[2] (FP64 atomics; the sums' last bits depend on the order). */
-__global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, const double* zref, double* out)
+ * out[2] (FP64 atomics; the sums' last bits depend on the order), the number
+ * of cells with |z - zref| > bound (or NaN) into out[3] and the number of wet
+ * cells into out[4] (integers below 2^53, exact in FP64). */
+__global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, const double* zref, double bound,
+                                                  double* out)
 {
     const HdDevG& d = dev_view(dv);
     const int N = d.N, M = d.M, W = d.W;
-    __shared__ double s1[256], s2[256];
+    __shared__ double s1[256], s2[256], s3[256], s4[256];
     __shared__ unsigned long long smx[256];
-    double v1 = 0, v2 = 0;
+    double v1 = 0, v2 = 0, nx = 0, nw = 0;
     unsigned long long mx = 0;
     const long n = (long)(N + 3) * W;
     for (long p = blockIdx.x * 256L + threadIdx.x; p < n; p += gridDim.x * 256L) {
@@ -2350,17 +2353,23 @@ __global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, 
         unsigned long long b = __double_as_longlong(dz);
         if (!(dz <= 1e300)) b = 0x7FF0000000000000ull; /* NaN counts as infinite */
         if (b > mx) mx = b;
+        if (!(dz <= bound)) nx += 1;
+        nw += 1;
         v1 += d.htaiz[p] + d.z[p];
         v2 += d.htaiz[p] + zref[p];
     }
     s1[threadIdx.x] = v1;
     s2[threadIdx.x] = v2;
+    s3[threadIdx.x] = nx;
+    s4[threadIdx.x] = nw;
     smx[threadIdx.x] = mx;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) {
             s1[threadIdx.x] += s1[threadIdx.x + k];
             s2[threadIdx.x] += s2[threadIdx.x + k];
+            s3[threadIdx.x] += s3[threadIdx.x + k];
+            s4[threadIdx.x] += s4[threadIdx.x + k];
             if (smx[threadIdx.x + k] > smx[threadIdx.x]) smx[threadIdx.x] = smx[threadIdx.x + k];
         }
         __syncthreads();
@@ -2369,6 +2378,8 @@ __global__ void __launch_bounds__(256) k_z_change(const HdDev* __restrict__ dv, 
         atomicMax((unsigned long long*)out, smx[0]);
         atomicAdd(out + 1, s1[0]);
         atomicAdd(out + 2, s2[0]);
+        atomicAdd(out + 3, s3[0]);
+        atomicAdd(out + 4, s4[0]);
     }
 }
 
@@ -3078,9 +3089,9 @@ void hd_launch_bed_extrapolate(const HdBatch& b, const double* snap, double f, h
     hipLaunchKernelGGL(k_bed_extrapolate, dim3((unsigned)(g < 4096 ? g : 4096), 1, b.n), dim3(256), 0, s, b.dv, snap, f);
 }
 
-void hd_launch_z_change(const HdBatch& b, const double* zref, double* out, hipStream_t s)
+void hd_launch_z_change(const HdBatch& b, const double* zref, double bound, double* out, hipStream_t s)
 {
     long n = (long)(b.N + 3) * (b.M + 3);
     long g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_z_change, dim3((unsigned)(g < 1024 ? g : 1024), 1, b.n), dim3(256), 0, s, b.dv, zref, out);
+    hipLaunchKernelGGL(k_z_change, dim3((unsigned)(g < 1024 ? g : 1024), 1, b.n), dim3(256), 0, s, b.dv, zref, bound, out);
 }

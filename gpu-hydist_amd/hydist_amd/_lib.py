@@ -25,7 +25,7 @@ EXPORTS = (
     "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
     "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
-    "hd_z_snapshot", "hd_z_change", "hd_thomas_batch_noalloc",
+    "hd_z_snapshot", "hd_z_change", "hd_z_check", "hd_thomas_batch_noalloc",
     "hd_k_Onetime_init", "hd_k_Find_Calculation_limits", "hd_k_Htuongdoi", "hd_k_preprocess_data",
     "hd_k_Update_Boundary_Value", "hd_k_UZSolver", "hd_k_VZSolver", "hd_k_UZSolver_calculate_matrix_coeff",
     "hd_k_VZSolver_calculate_matrix_coeff", "hd_k_Normalize", "hd_k_solveV", "hd_k_solveU", "hd_k_Reset_states",
@@ -108,6 +108,7 @@ def load() -> ctypes.CDLL:
         "hd_copy_state": (ci, [vp, vp]),
         "hd_z_snapshot": (ci, [vp]),
         "hd_z_change": (ci, [vp, dp, dp, dp]),
+        "hd_z_check": (ci, [vp, cd, dp, dp, dp, dp]),
         "hd_bed_extrapolate": (ci, [vp, cd]),
         "hd_sediment_run": (ci, [vp, cd, ci, po, dp]),
         "hd_ensemble_create": (ci, [ctypes.POINTER(vp), ci, ctypes.POINTER(vp)]),
@@ -266,6 +267,14 @@ class Solver:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self._chk(load().hd_z_change(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def z_check(self, bound):
+        """(max |z - z_snapshot|, fraction of wet cells above `bound`, wet
+        volume now, at the snapshot)."""
+        a, f, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._chk(load().hd_z_check(self.h, bound, ctypes.byref(a), ctypes.byref(f), ctypes.byref(b),
+                                    ctypes.byref(c)))
+        return a.value, f.value, b.value, c.value
 
     def copy_state_from(self, other):
         """Overwrite this context's device state with other's (same case)."""

@@ -11,10 +11,20 @@ change every step).  The accelerated path, per macro step of K steps:
 2. the sediment half of K steps (FS transport, bed change) runs on that
    frozen flow (Solver.sediment_run);
 3. every `check_every` network calls one plain FP64 ADI step runs from the
-   surrogate state; its change of the water level over the wet cells (the
-   state's departure from the FP64 dynamics) and the relative change of the
-   wet volume (mass balance) are compared with the bounds; past either bound
-   the driver reverts to plain ADI for `fallback_macro` macro steps.
+   surrogate state; the fraction of wet cells whose water level that step
+   moves by more than `z_bound` (the state's departure from the FP64
+   dynamics: a plain step moves z by about 1 mm except at the wetting and
+   drying cells, up to about 1 cm) and the relative change of the wet volume
+   (mass balance) are compared with the bounds; past either bound the
+   driver reverts to plain ADI for `fallback_macro` macro steps.
+
+The flow does not depend on the bed: the reference accumulates the bed
+change dH (Sediment_Transport.cu:504) and never feeds it back into the depth
+h, so the bed enters the network's input only scaled out (s_b).  That also
+keeps the bank cells whose dH runs away (a one-step velocity spike of tens
+of m/s over a few centimetres of water drives FS far above 1, and the
+hindered settling velocity wss (1 - FS)^4 of Sediment_Transport.cu:230 then
+grows as FS^4) from spoiling the prediction of their neighbours.
 
 Training data are (state, state + K steps) pairs from plain ADI runs of the
 river with seeded inflow variants, kept on the device (hd_cnn_data_alloc /
@@ -31,7 +41,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import cnn, synth
-from ._lib import Solver
+from ._lib import HydistError, Solver
 from .case import BED_EVERY_STEP
 
 
@@ -51,7 +61,12 @@ def forcing(case, t: float):
     return series_at(a["bc_down"], t), series_at(a["bc_up"], t)
 
 
-SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=1.0, s_q=3000.0, s_h=10.0, d_u=0.02, d_v=0.02, d_z=0.01)
+# wet runs per line allowed (the header's segment_limit; the library's
+# largest): over days the wet mask fragments past the river's 20, more in
+# some runs than in others
+SEGMENT_LIMIT = 24
+
+SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=1e30, s_q=3000.0, s_h=10.0, d_u=0.02, d_v=0.02, d_z=0.01)
 
 
 def river_variant(k: int, scale: float = 1.0, hours: float = 30.0, seed: int = 50):
@@ -60,7 +75,7 @@ def river_variant(k: int, scale: float = 1.0, hours: float = 30.0, seed: int = 5
     rng = np.random.default_rng(seed + k)
     qm = 3000.0 * scale * float(rng.uniform(0.75, 1.25))
     qa = qm * float(rng.uniform(0.15, 0.4))
-    c = synth.config(3, scale=scale, hours=hours, q_mean=qm, q_amp=qa)
+    c = synth.config(3, scale=scale, hours=hours, q_mean=qm, q_amp=qa, SL=SEGMENT_LIMIT)
     c.bed_mode = BED_EVERY_STEP
     c.name = f"{c.name}_v{k}"
     return c
@@ -145,6 +160,7 @@ class RunReport:
     checks: int = 0
     fallbacks: int = 0
     adi_steps: int = 0
+    accepted: int = 0
 
 
 def plain_run(case, t0: float, t1: float, g: Solver, log=None) -> RunReport:
@@ -174,15 +190,18 @@ def wet_volume(g: Solver) -> float:
 
 
 def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: int, check_every: int = 10,
-                  z_bound: float = 5e-3, mass_bound: float = 1e-4, fallback_macro: int = 5,
-                  morph_substeps: int = 0) -> RunReport:
+                  z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, fallback_macro: int = 5,
+                  morph_substeps: int = 0, log=None) -> RunReport:
     """Bed-change phase from t0 to t1 with the network standing in for the
-    hydrodynamics (module docstring).  The last state that passed a check is
-    kept on the device (a clone); a failed check restores it and runs plain
-    ADI from there for `fallback_macro` macro steps.  morph_substeps = m > 0:
-    of the K sediment steps on the predicted flow only m run and the bed
-    change of the last of them is repeated K - m times (morphological
-    acceleration over the quasi-steady flow, hd_bed_extrapolate)."""
+    hydrodynamics (module docstring).  A check passes when at most a
+    fraction `z_frac` of the wet cells move by more than `z_bound` in the
+    FP64 check step and the wet volume changes by at most `mass_bound`
+    (relative).  The last state that passed a check is kept on the device (a
+    clone); a failed check restores it and runs plain ADI from there for
+    `fallback_macro` macro steps.  morph_substeps = m > 0: of the K sediment
+    steps on the predicted flow only m run and the bed change of the last of
+    them is repeated K - m times (morphological acceleration over the
+    quasi-steady flow, hd_bed_extrapolate)."""
     op = case.opts()
     rep = RunReport(0, 0, 0)
     t = t0
@@ -223,9 +242,14 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
             g.z_snapshot()
             t = g.run(t, 1, op)
             rep.adi_steps += 1
-            dz, v1, v0 = g.z_change()
+            dz, frac, v1, v0 = g.z_check(z_bound)
             dv = abs(v1 - v0) / max(abs(v0), 1e-30)
-            if dz <= z_bound and dv <= mass_bound:     # NaN fails too
+            ok = frac <= z_frac and dv <= mass_bound      # NaN fails too
+            if log:
+                log(f"    check at {t / 3600.0:.2f} h: max dz {dz:.3e} m, over {z_bound:g} m {frac:.2e}, "
+                    f"dV {dv:.2e} -> {'pass' if ok else 'fail'}")
+            if ok:
+                rep.accepted += 1
                 good.copy_state_from(g)
                 t_good = t
             else:
@@ -241,14 +265,39 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
     return rep
 
 
-def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 3, batch: int = 4,
-             lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, log=print) -> dict:
+def check_statistics(case, g: Solver, t: float, z_bound: float = 5e-3):
+    """What the surrogate's check measures, for one plain FP64 step from the
+    current state (left unchanged): (max |dz|, fraction of wet cells over
+    z_bound, relative wet-volume change)."""
+    c = g.clone()
+    c.z_snapshot()
+    c.run(t, 1, case.opts())
+    dz, frac, v1, v0 = c.z_check(z_bound)
+    c.close()
+    return dz, frac, abs(v1 - v0) / max(abs(v0), 1e-30)
+
+
+def bed_difference(d_ref: np.ndarray, d: np.ndarray, wet: np.ndarray) -> dict:
+    """Bed-change difference over the wet cells: max, 99th and 99.9th
+    percentiles of |d - d_ref| and the fraction of cells within 1 mm."""
+    e = np.abs(d[wet] - d_ref[wet])
+    e = np.where(np.isfinite(e), e, np.inf)
+    return {"max_m": float(e.max()), "p99_m": float(np.percentile(e, 99)),
+            "p999_m": float(np.percentile(e, 99.9)), "frac_within_1mm": float((e <= 1e-3).mean())}
+
+
+def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 6, batch: int = 4,
+             lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, check_every: int = 10,
+             z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
     bed-change run with the plain one on the config-3 river (bed change every
     step, from hour 2): simulated hours per wall second of both, and the bed
-    (dH) difference after `days` days."""
+    (dH) difference after one day and after `days` days.  Beside them, the
+    FP64 method against itself: a twin run whose inflow series is scaled by
+    1 + 1e-12, the smallest difference any other method could have, and the
+    check's statistics on plain steps."""
     warm = 2.0
-    case = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0)
+    case = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0, SL=SEGMENT_LIMIT)
     case.bed_mode = BED_EVERY_STEP
     H, W = case.N + 3, case.M + 3
     net = cnn.Network(H, W, batch=batch, layers=layers, seed=1)
@@ -262,37 +311,89 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
     pers = persistence_loss(net, va)
     log(f"  validation loss {val:.3e} (persistence {pers:.3e})")
     tb = warm * 3600.0
-    te = tb + days * 86400.0
+    marks = [tb + min(1.0, days) * 86400.0, tb + days * 86400.0]
     op = case.opts()
-    g1 = Solver(case)
-    g1.init()
-    g1.run(0.0, int(round(tb / case.dT)), op)
-    plain = plain_run(case, tb, te, g1, log=log)
-    log(f"  plain ADI: {plain.sim_hours:.1f} h in {plain.wall_s:.1f} s")
-    d1, kh = g1.get("dH"), g1.get("khouot")
-    wet = kh == 0
+    w0 = int(round(tb / case.dT))
+
+    def fresh(c):
+        g = Solver(c)
+        g.init()
+        g.run(0.0, w0, c.opts())
+        return g
+
+    g1 = fresh(case)
+    plain_beds, plain_wall, t = [], 0.0, tb
+    for m in marks:
+        r = plain_run(case, t, m, g1, log=log)
+        plain_wall += r.wall_s
+        t = m
+        plain_beds.append((g1.get("dH"), g1.get("khouot") == 0))
+    plain_rate = (marks[-1] - tb) / 3600.0 / plain_wall
+    log(f"  plain ADI: {(marks[-1] - tb) / 3600.0:.1f} h in {plain_wall:.1f} s")
+    twin = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0, SL=SEGMENT_LIMIT)
+    twin.bed_mode = BED_EVERY_STEP
+    for key in ("bc_left", "bc_down"):
+        if key in twin.arrays and np.any(twin.arrays[key]):
+            twin.arrays[key] = twin.arrays[key] * (1.0 + 1e-12)
+            break
+    g3 = fresh(twin)
+    t, stats, twin_diff, twin_stop = tb, [], [None] * len(marks), None
+    step6 = 6 * 3600.0
+    while t < marks[-1] - 1e-9:
+        try:
+            t = g3.run(t, int(round(min(step6, marks[-1] - t) / case.dT)), twin.opts())
+        except HydistError as e:          # e.g. more wet runs on a line than the limit
+            twin_stop = f"stopped after {(t - tb) / 3600.0:.0f} h: {e}"
+            log(f"  twin {twin_stop}")
+            break
+        stats.append(check_statistics(twin, g3, t, z_bound))
+        for k, m in enumerate(marks):
+            if abs(t - m) < 1e-6:
+                twin_diff[k] = bed_difference(plain_beds[k][0], g3.get("dH"), plain_beds[k][1])
+    g3.close()
+    st = np.array(stats)
+    plain_checks = {"max_dz_m": float(st[:, 0].max()), "frac_over_max": float(st[:, 1].max()),
+                    "dV_max": float(st[:, 2].max()), "samples": len(stats), "every_h": 6}
+    log(f"  plain check statistics: {plain_checks}")
     modes = {}
-    for label, m in (("full_sediment_steps", 0), (f"morph_{morph}", morph)):
-        g2 = Solver(case)
-        g2.init()
-        g2.run(0.0, int(round(tb / case.dT)), op)
-        sur = surrogate_run(case, tb, te, g2, net, K, morph_substeps=m)
-        d2 = g2.get("dH")
-        err = float(np.abs(d1[wet] - d2[wet]).max())
-        rms = float(np.sqrt(np.mean((d1[wet] - d2[wet]) ** 2)))
-        log(f"  surrogate ({label}): {sur.sim_hours:.1f} h in {sur.wall_s:.1f} s, {sur.surrogate_calls} calls, "
-            f"{sur.fallbacks} fallbacks, bed error max {err:.3e} m rms {rms:.3e} m")
-        modes[label] = {"sim_hours_per_wall_s": sur.sim_hours_per_wall_s,
-                        "speedup": sur.sim_hours_per_wall_s / plain.sim_hours_per_wall_s,
-                        "bed_error_max_m": err, "bed_error_rms_m": rms, "surrogate_calls": sur.surrogate_calls,
-                        "checks": sur.checks, "fallbacks": sur.fallbacks, "adi_steps": sur.adi_steps}
+    for label, msub in (("full_sediment_steps", 0), (f"morph_{morph}", morph)):
+        g2 = fresh(case)
+        t, wall, diffs, calls, acc, fb, adi, checks = tb, 0.0, [], 0, 0, 0, 0, 0
+        stop = None
+        for k, m in enumerate(marks):
+            try:
+                sur = surrogate_run(case, t, m, g2, net, K, check_every=check_every, z_bound=z_bound,
+                                    z_frac=z_frac, mass_bound=mass_bound, morph_substeps=msub)
+            except HydistError as e:
+                stop = f"stopped in the segment ending at {(m - tb) / 3600.0:.0f} h: {e}"
+                log(f"  surrogate ({label}) {stop}")
+                break
+            t = m
+            wall += sur.wall_s
+            calls += sur.surrogate_calls
+            acc += sur.accepted
+            fb += sur.fallbacks
+            adi += sur.adi_steps
+            checks += sur.checks
+            diffs.append(bed_difference(plain_beds[k][0], g2.get("dH"), plain_beds[k][1]))
+        if stop:
+            modes[label] = {"stopped": stop, "bed": diffs}
+            g2.close()
+            continue
+        rate = (marks[-1] - tb) / 3600.0 / wall
+        modes[label] = {"sim_hours_per_wall_s": rate, "speedup": rate / plain_rate, "surrogate_calls": calls,
+                        "checks": checks, "accepted": acc, "fallbacks": fb, "adi_steps": adi,
+                        "bed_1d": diffs[0], "bed_end": diffs[-1]}
+        log(f"  surrogate ({label}): {wall:.1f} s, speed-up {rate / plain_rate:.2f}, {calls} calls, "
+            f"{acc}/{checks} checks passed, bed at the end {diffs[-1]}")
         g2.close()
-    bed_change = float(np.abs(d1[wet]).max())
+    g1.close()
     net.close()
     return {
         "workload": f"{case.name} bed change {days:g} d (every step), K={K}",
-        "plain_sim_hours_per_wall_s": plain.sim_hours_per_wall_s,
-        "bed_change_max_m": bed_change, "modes": modes,
+        "plain_sim_hours_per_wall_s": plain_rate, "plain_check_statistics": plain_checks,
+        "twin_bed_1d": twin_diff[0], "twin_bed_end": twin_diff[-1], "twin_stopped": twin_stop, "modes": modes,
+        "bounds": {"z_bound_m": z_bound, "z_frac": z_frac, "mass_bound": mass_bound, "check_every": check_every},
         "train": {"samples": n, "epochs": epochs, "loss": hist, "val_loss": val,
                   "val_loss_persistence": pers, "gen_s": gen_s, "train_s": train_s},
     }
@@ -307,11 +408,17 @@ if __name__ == "__main__":
     ap.add_argument("--variants", type=int, default=8)
     ap.add_argument("--pairs", type=int, default=250)
     ap.add_argument("--K", type=int, default=60)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--days", type=float, default=7.0)
     ap.add_argument("--morph", type=int, default=6)
+    ap.add_argument("--layers", type=int, default=8)
+    ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--z-bound", type=float, default=5e-3)
+    ap.add_argument("--z-frac", type=float, default=1e-3)
+    ap.add_argument("--mass-bound", type=float, default=1e-4)
     a = ap.parse_args()
     r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
-                 days=a.days, morph=a.morph, log=lambda s: print(s, flush=True))
+                 days=a.days, layers=a.layers, morph=a.morph, check_every=a.check_every, z_bound=a.z_bound,
+                 z_frac=a.z_frac, mass_bound=a.mass_bound, log=lambda s: print(s, flush=True))
     print(json.dumps(r))

@@ -188,6 +188,13 @@ int hd_bed_extrapolate(hd_ctx* c, double factor);
  * at the snapshot (the wet volume per unit cell area). */
 int hd_z_snapshot(hd_ctx* c);
 int hd_z_change(hd_ctx* c, double* max_abs_dz, double* wet_volume, double* wet_volume_snap);
+/* hd_z_change with a bound: also the fraction of the wet cells whose
+ * |z - z_saved| exceeds `bound` (a NaN counts as exceeding).  The largest
+ * change sits at wetting and drying cells, where one plain step moves z by
+ * up to about a centimetre; the fraction over the bound is the measure that
+ * plain steps pass. */
+int hd_z_check(hd_ctx* c, double bound, double* max_abs_dz, double* frac_over, double* wet_volume,
+               double* wet_volume_snap);
 
 /* A second context with the same case and a copy of the whole device state
  * (fields, masks, segment and run tables, counters): for what-if runs and

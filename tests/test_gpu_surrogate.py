@@ -43,9 +43,9 @@ def test_surrogate_driver_checks_and_falls_back():
     g.init()
     tb = 2.0 * 3600.0
     g.run(0.0, int(round(tb / case.dT)), op)
-    # an impossible z bound: every check fails, the last verified state comes
-    # back and plain ADI takes over from it
-    rep = S.surrogate_run(case, tb, tb + 30 * K * case.dT, g, net, K, check_every=2, z_bound=-1.0,
+    # an impossible bound (a negative fraction of cells): every check fails,
+    # the last verified state comes back and plain ADI takes over from it
+    rep = S.surrogate_run(case, tb, tb + 30 * K * case.dT, g, net, K, check_every=2, z_frac=-1.0,
                           fallback_macro=3)
     assert rep.checks >= 1 and rep.fallbacks == rep.checks
     assert rep.adi_steps >= 3 * K * rep.fallbacks
