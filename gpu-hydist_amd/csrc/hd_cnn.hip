@@ -49,16 +49,23 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f3
 }
 
 /* Y = conv3x3(X) (+ bias) with epilogue EPI; images [nimg][H][W][C]. */
+/* With a tile list (tl != nullptr) workgroup k covers tile tl[k] for k <
+ * *tn and does nothing beyond; without one, workgroup k covers tile k. */
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
                                               const float* __restrict__ bias, int nimg, int H, int W,
                                               const bf16* __restrict__ res, bf16* __restrict__ Y,
-                                              bf16* __restrict__ R, float* __restrict__ Yf)
+                                              bf16* __restrict__ R, float* __restrict__ Yf,
+                                              const int* __restrict__ tl, const int* __restrict__ tn)
 {
     constexpr int NB = COUT / 16;
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int tilesW = (W + 63) >> 6, tilesH = (H + 3) >> 2;
     int t = blockIdx.x;
+    if (tl) {
+        if ((int)blockIdx.x >= *tn) return;
+        t = tl[blockIdx.x];
+    }
     const int tw = t % tilesW;
     t /= tilesW;
     const int th = t % tilesH;
@@ -361,6 +368,35 @@ __global__ void __launch_bounds__(256) k_unpack(const HdDev* __restrict__ dv, hd
     }
 }
 
+/* The forward tiles (4 rows x 64 columns of the image) that lie within
+ * `rad` cells (Chebyshev distance) of a wet cell of [1,N]x[1,M]: one wave
+ * per tile scans the tile grown by rad on every side; the list order is
+ * arbitrary (each tile's outputs do not depend on it).  *count must be zero
+ * on entry. */
+__global__ void __launch_bounds__(64) k_active_tiles(const HdDev* __restrict__ dv, int rad, int* __restrict__ list,
+                                                     int* __restrict__ count)
+{
+    const HdDevG& d = reinterpret_cast<const HdDevG*>(dv)[0];
+    const int N = d.N, M = d.M, W = d.W, H = N + 3;
+    const int tilesW = (W + 63) >> 6;
+    const int t = blockIdx.x, tw = t % tilesW, th = t / tilesW;
+    if (th * 4 >= H) return;
+    const int i0 = max(th * 4 - rad, 1), i1 = min(th * 4 + 3 + rad, N);
+    const int j0 = max(tw * 64 - rad, 1), j1 = min(tw * 64 + 63 + rad, M);
+    bool any = false;
+    if (i0 <= i1 && j0 <= j1) {
+        const int wj = j1 - j0 + 1;
+        const long n = (long)(i1 - i0 + 1) * wj;
+        for (long k = threadIdx.x; k < n && !any; k += 64) {
+            const int i = i0 + (int)(k / wj), j = j0 + (int)(k % wj);
+            any = d.khouot[(long)i * W + j] == 0;
+        }
+    }
+    if (__any(any) && threadIdx.x == 0) list[atomicAdd(count, 1)] = t;
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
 __global__ void k_mfma_check(const bf16* A, const bf16* B, float* D)
 {
     const int l = threadIdx.x;
@@ -409,6 +445,11 @@ struct hd_cnn {
     /* sample store */
     int nsamp = 0;
     bf16* sx = nullptr;                 /* [n][P][8] */
+    /* tile-sparse inference (hd_cnn_predict_into_ctx) */
+    bool sparse = true;
+    int* tlist = nullptr;               /* [tiles of one image] */
+    int* tcount = nullptr;              /* [1] */
+    const int* tl = nullptr;            /* the list the next forward uses (nullptr: all tiles) */
     float *sbase = nullptr, *stgt = nullptr, *smsk = nullptr;
     std::string err;
 };
@@ -465,7 +506,7 @@ static void conv(hd_cnn* n, const bf16* X, const Layer& y, bool transposed, int 
 {
     int tiles = ((n->W + 63) / 64) * ((n->H + 3) / 4) * nimg;
     hipLaunchKernelGGL((k_conv<CIN, COUT, EPI>), dim3(tiles), dim3(256), 0, n->stream, X, transposed ? y.wT16 : y.w16,
-                       y.b32, nimg, n->H, n->W, res, Y, R, Yf);
+                       y.b32, nimg, n->H, n->W, res, Y, R, Yf, n->tl, n->tcount);
 }
 
 template <int CIN, int COUT>
@@ -608,7 +649,9 @@ int hd_cnn_create(int device, int H, int W, int batch, int layers, uint64_t seed
         (rc = calloc_dev(n, (void**)&n->tgt, PB * 3 * 4)) || (rc = calloc_dev(n, (void**)&n->msk, PB * 4)) ||
         (rc = calloc_dev(n, (void**)&n->pred, PB * 3 * 4)) || (rc = calloc_dev(n, (void**)&n->out, PB * kOutPad * 4)) ||
         (rc = calloc_dev(n, (void**)&n->dA, PB * kWidth * 4)) || (rc = calloc_dev(n, (void**)&n->dZ, PB * kWidth * 2)) ||
-        (rc = calloc_dev(n, (void**)&n->loss, 8))) {
+        (rc = calloc_dev(n, (void**)&n->loss, 8)) ||
+        (rc = calloc_dev(n, (void**)&n->tlist, (size_t)((W + 63) / 64) * ((H + 3) / 4) * 4)) ||
+        (rc = calloc_dev(n, (void**)&n->tcount, 4))) {
         hd_cnn_destroy(n);
         return rc;
     }
@@ -632,6 +675,13 @@ int hd_cnn_set_scales(hd_cnn* n, const hd_cnn_scales* s)
 {
     if (!n || !s || !(s->d_u > 0) || !(s->d_v > 0) || !(s->d_z > 0)) return HD_EINVAL;
     n->sc = *s;
+    return HD_OK;
+}
+
+int hd_cnn_set_sparse(hd_cnn* n, int on)
+{
+    if (!n) return HD_EINVAL;
+    n->sparse = on != 0;
     return HD_OK;
 }
 
@@ -832,7 +882,19 @@ int hd_cnn_predict_into_ctx(hd_cnn* n, hd_ctx* c, float q, float zb)
     bf16* xs = (bf16*)n->dZ;   /* scratch for the 8-channel sample (the gradient buffer is free here) */
     hipLaunchKernelGGL(k_pack, dim3(nblk(n->P)), dim3(256), 0, n->stream, dv, n->sc, q, zb, xs, n->base, nullptr);
     hipLaunchKernelGGL(k_expand_x, dim3(nblk(n->P)), dim3(256), 0, n->stream, xs, n->P, n->x0);
+    if (n->sparse) {
+        /* Only the tiles within L - 1 cells of a wet cell: the first layer
+         * reads the whole input image, and a layer's value at a cell is the
+         * dense one wherever its 3x3 neighbours' values were, so with every
+         * cell within L - 1 of a wet cell computed the network's output at
+         * the wet cells is that of the dense pass, bit for bit. */
+        int tiles = ((n->W + 63) / 64) * ((n->H + 3) / 4);
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, n->stream, n->tcount);
+        hipLaunchKernelGGL(k_active_tiles, dim3(tiles), dim3(64), 0, n->stream, dv, n->L - 1, n->tlist, n->tcount);
+        n->tl = n->tlist;
+    }
     int rc = forward(n, 1);
+    n->tl = nullptr;
     if (rc) return rc;
     hipLaunchKernelGGL(k_unpack, dim3(nblk(n->P)), dim3(256), 0, n->stream, dv, n->sc, n->out);
     CHK(n, hipGetLastError());

@@ -19,6 +19,7 @@ EXPORTS = (
     "hd_cnn_get_layer", "hd_cnn_set_layer", "hd_cnn_get_grad", "hd_cnn_load_batch", "hd_cnn_forward",
     "hd_cnn_backward", "hd_cnn_adam", "hd_cnn_train_step", "hd_cnn_data_alloc", "hd_cnn_capture",
     "hd_cnn_train_samples", "hd_cnn_eval_samples", "hd_cnn_predict_into_ctx", "hd_cnn_mfma_check",
+    "hd_cnn_set_sparse",
 )
 
 
@@ -55,6 +56,7 @@ def lib():
             "hd_cnn_eval_samples": (ci, [vp, ip, ci, dp]),
             "hd_cnn_predict_into_ctx": (ci, [vp, vp, cf, cf]),
             "hd_cnn_mfma_check": (ci, [ci, fp, fp, fp]),
+            "hd_cnn_set_sparse": (ci, [vp, ci]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -187,3 +189,8 @@ class Network:
 
     def predict_into(self, solver, q=0.0, zb=0.0):
         self._chk(lib().hd_cnn_predict_into_ctx(self.h, solver.h, q, zb))
+
+    def set_sparse(self, on: bool):
+        """Tile-sparse prediction (the default): only the image tiles within
+        layers - 1 cells of a wet cell; same outputs at the wet cells."""
+        self._chk(lib().hd_cnn_set_sparse(self.h, 1 if on else 0))

@@ -288,7 +288,8 @@ def bed_difference(d_ref: np.ndarray, d: np.ndarray, wet: np.ndarray) -> dict:
 
 def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 6, batch: int = 4,
              lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, check_every: int = 10,
-             z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log=print) -> dict:
+             z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log_checks: bool = False,
+             log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
     bed-change run with the plain one on the config-3 river (bed change every
     step, from hour 2): simulated hours per wall second of both, and the bed
@@ -363,7 +364,8 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
         for k, m in enumerate(marks):
             try:
                 sur = surrogate_run(case, t, m, g2, net, K, check_every=check_every, z_bound=z_bound,
-                                    z_frac=z_frac, mass_bound=mass_bound, morph_substeps=msub)
+                                    z_frac=z_frac, mass_bound=mass_bound, morph_substeps=msub,
+                                    log=log if log_checks else None)
             except HydistError as e:
                 stop = f"stopped in the segment ending at {(m - tb) / 3600.0:.0f} h: {e}"
                 log(f"  surrogate ({label}) {stop}")
@@ -417,8 +419,10 @@ if __name__ == "__main__":
     ap.add_argument("--z-bound", type=float, default=5e-3)
     ap.add_argument("--z-frac", type=float, default=1e-3)
     ap.add_argument("--mass-bound", type=float, default=1e-4)
+    ap.add_argument("--log-checks", action="store_true")
     a = ap.parse_args()
     r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
                  days=a.days, layers=a.layers, morph=a.morph, check_every=a.check_every, z_bound=a.z_bound,
-                 z_frac=a.z_frac, mass_bound=a.mass_bound, log=lambda s: print(s, flush=True))
+                 z_frac=a.z_frac, mass_bound=a.mass_bound, log_checks=a.log_checks,
+                 log=lambda s: print(s, flush=True))
     print(json.dumps(r))

@@ -102,6 +102,11 @@ int hd_cnn_eval_samples(hd_cnn* n, const int* host_idx, int b, double* loss);
  * cells (khouot == 0) back into the context (u, v, z, t_u, t_v, t_z), then
  * refresh H_moi / Htdu / Htdv (hd_flow_refresh). */
 int hd_cnn_predict_into_ctx(hd_cnn* n, hd_ctx* c, float q, float zb);
+/* on != 0 (the default): hd_cnn_predict_into_ctx runs the convolutions only
+ * over the 4 x 64 image tiles within L - 1 cells of a wet cell, which leaves
+ * the outputs at the wet cells those of the dense pass, bit for bit (the
+ * river covers about a tenth of its grid); on == 0: every tile. */
+int hd_cnn_set_sparse(hd_cnn* n, int on);
 
 /* One-tile check of the MFMA operand layout the kernels use: D = A.B for
  * A [16][32], B [32][16] (host FP32, rounded to BF16), D [16][16] FP32. */

@@ -62,3 +62,33 @@ def test_surrogate_driver_checks_and_falls_back():
     g2.run(0.0, int(round(tb / case.dT)), op)
     rep2 = S.surrogate_run(case, tb, tb + 6 * K * case.dT, g2, net, K, check_every=100, morph_substeps=4)
     assert rep2.surrogate_calls == 6 and abs(rep2.sim_hours - 6 * K * case.dT / 3600.0) < 1e-9
+
+
+def test_surrogate_accepted_at_real_bounds():
+    """With the stated bounds (at most 0.1 % of the wet cells moved by more
+    than 5 mm in the FP64 check step, wet volume within 1e-4) a network
+    trained on the river's own pairs is accepted on the small river at
+    K = 10, and the bed it drives stays close to the plain FP64 run's: 99 %
+    of the wet cells within 15 mm and 90 % within 1 mm after an hour of bed
+    change every step (the network's outputs are BF16, so not bit-exact)."""
+    K10 = 10
+    net = small_net(80)
+    n = S.build_dataset(net, variants=2, pairs=40, K=K10, scale=SCALE, log=lambda s: None)
+    S.train(net, n, epochs=20, batch=2, lr=1e-3, val_frac=0.1, log=lambda s: None)
+    case = synth.config(3, scale=SCALE, hours=6.0, SL=S.SEGMENT_LIMIT)
+    case.bed_mode = BED_EVERY_STEP
+    op = case.opts()
+    plain = Solver(case)
+    plain.init()
+    tb = 2.0 * 3600.0
+    plain.run(0.0, int(round(tb / case.dT)), op)
+    sur = plain.clone()
+    S.plain_run(case, tb, tb + 3600.0, plain)
+    rep = S.surrogate_run(case, tb, tb + 3600.0, sur, net, K10, check_every=2, z_bound=5e-3, z_frac=1e-3,
+                          mass_bound=1e-4)
+    assert rep.checks >= 20 and rep.accepted >= 0.75 * rep.checks, rep
+    assert rep.surrogate_calls >= 40 and abs(rep.sim_hours - 1.0) < 1e-9
+    wet = plain.get("khouot") == 0
+    d = S.bed_difference(plain.get("dH"), sur.get("dH"), wet)
+    assert d["p99_m"] <= 0.015 and d["frac_within_1mm"] >= 0.9, d
+    assert np.isfinite(sur.get("z")[wet]).all()
