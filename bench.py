@@ -16,10 +16,11 @@ seeded rivers, one launch per kernel over all of them).
 
     python bench.py [--gpus 1] [--steps 720] [--warmup 20] [--no-cpu] [--no-ensemble]
 
---workload sediment | ensemble | cnn prints the line of another subsystem
+--workload sediment | ensemble | cnn | surrogate prints the line of another subsystem
 instead (the river with suspended sediment and bed change every step; the
 ensemble alone, with per-kernel times; the surrogate network's forward pass
-and training step), for profiling each subsystem with rocprofv3.
+and training step; the surrogate-accelerated bed change against plain FP64
+ADI and an FP64 twin), for profiling each subsystem with rocprofv3.
 """
 from __future__ import annotations
 
@@ -116,11 +117,15 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--ensemble-steps", type=int, default=720)
     p.add_argument("--no-ensemble", action="store_true")
-    p.add_argument("--workload", default="river", choices=("river", "sediment", "ensemble", "cnn"),
+    p.add_argument("--workload", default="river", choices=("river", "sediment", "ensemble", "cnn", "surrogate"),
                    help="river: the headline line (default); the others print their own line "
                         "(sediment: the river with suspended sediment and bed change every step; "
-                        "ensemble: the 64-river batch; cnn: the surrogate network's passes)")
+                        "ensemble: the 64-river batch; cnn: the surrogate network's passes; surrogate: the "
+                        "surrogate-accelerated bed change against plain FP64 ADI, about two minutes)")
     p.add_argument("--cnn-reps", type=int, default=20)
+    p.add_argument("--sur-pairs", type=int, default=60, help="surrogate: training pairs per inflow variant")
+    p.add_argument("--sur-epochs", type=int, default=6)
+    p.add_argument("--sur-days", type=float, default=1.0, help="surrogate: days of bed change compared")
     return p.parse_args()
 
 
@@ -453,13 +458,40 @@ def cnn_line(a):
             "passes": out}
 
 
+def surrogate_line(a):
+    """Config 5 in a bounded form: train the surrogate on (state, state + 60
+    steps) pairs from 8 inflow variants of the config-3 river, then run a day
+    of bed change every step three ways from the same state at hour 2 --
+    plain FP64 ADI, the surrogate with an FP64 check after every call
+    (sediment every step, and with morphological factor 6), and an FP64 twin
+    whose inflow is scaled by 1 + 1e-12 -- and report the simulated hours per
+    wall second against plain ADI with the bed difference of each."""
+    from hydist_amd import surrogate
+    r = surrogate.evaluate(scale=1.0, variants=8, pairs=a.sur_pairs, K=60, epochs=a.sur_epochs, days=a.sur_days,
+                           check_every=1, log=lambda m: print(m, file=sys.stderr, flush=True))
+    modes = r["modes"]
+    best = max(modes, key=lambda k: modes[k].get("speedup", 0.0))
+    m = modes[best]
+    return {"metric": "surrogate speed-up (simulated hours per wall second vs plain FP64 ADI) at the stated bed "
+                      "accuracy", "value": m.get("speedup"), "unit": "x", "n_gpus": 1, "higher_is_better": True,
+            "vs_baseline": None, "dtype": "bf16+f64", "data": "synthetic",
+            "config": {"workload": r["workload"], "mode": best, "K": 60, "samples": r["train"]["samples"],
+                       "epochs": a.sur_epochs, "days": a.sur_days},
+            "plain_sim_hours_per_wall_s": r["plain_sim_hours_per_wall_s"],
+            "bed_error_vs_plain": m.get("bed_end"), "checks": m.get("checks"), "accepted": m.get("accepted"),
+            "bed_floor_fp64_twin": r["twin_bed_end"], "plain_check_statistics": r["plain_check_statistics"],
+            "bounds": r["bounds"], "val_loss": r["train"]["val_loss"],
+            "val_loss_persistence": r["train"]["val_loss_persistence"], "modes": modes}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
-    line = {"river": river_line, "sediment": sediment_line, "ensemble": ensemble_line, "cnn": cnn_line}[a.workload](a)
+    line = {"river": river_line, "sediment": sediment_line, "ensemble": ensemble_line, "cnn": cnn_line,
+            "surrogate": surrogate_line}[a.workload](a)
     print(json.dumps(line))
 
 
