@@ -117,6 +117,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--ensemble-steps", type=int, default=720)
     p.add_argument("--no-ensemble", action="store_true")
+    p.add_argument("--no-tune", action="store_true", help="keep the default launch geometry (no autotuning)")
     p.add_argument("--workload", default="river", choices=("river", "sediment", "ensemble", "cnn", "surrogate"),
                    help="river: the headline line (default); the others print their own line "
                         "(sediment: the river with suspended sediment and bed change every step; "
@@ -221,7 +222,7 @@ def cpu_host():
     return os.cpu_count(), cpu_model()
 
 
-def ensemble_bench(nsteps, profile=False):
+def ensemble_bench(nsteps, profile=False, tune=True):
     """The batched-ensemble workload: 64 seeded meandering rivers (sides
     256-1024, own discharge, roughness, tide; sediment on), one launch per
     kernel over all of them, nsteps steps (720 = one simulated hour)."""
@@ -252,6 +253,11 @@ def ensemble_bench(nsteps, profile=False):
         ms.append(g)
     setup = time.perf_counter() - t0
     op = cases[0].opts()
+    launch = None
+    if tune:      # launch geometry for the batch: from the cache, or timed with the states restored after
+        from hydist_amd import tuning
+        launch = tuning.tune_ensemble(ms, 0.0, op, steps=10)
+        launch.pop("key", None)
     ens = Ensemble(ms)
     t = ens.run(0.0, 2, op)                      # capture the step graphs
     for m in ms:
@@ -279,7 +285,8 @@ def ensemble_bench(nsteps, profile=False):
            "steps": nsteps, "sim_hours": nsteps * cases[0].dT / 3600.0, "value": rate,
            "unit": "wet-cell-steps/s", "ms_per_step": el * 1e3 / nsteps,
            "mean_wet_cells": wet_steps / nsteps, "members_finite": finite,
-           "hbm_frac_step_algorithmic": rate * B_STEP_SED / HBM_PEAK, "setup_s": round(setup, 1)}
+           "hbm_frac_step_algorithmic": rate * B_STEP_SED / HBM_PEAK, "setup_s": round(setup, 1),
+           "launch": launch}
     if profile:
         out["kt"] = kt
     ens.close()
@@ -288,15 +295,22 @@ def ensemble_bench(nsteps, profile=False):
     return out
 
 
-def timed_window(case, opts, warmup, steps):
-    """Warm-up steps, then `steps` steps replayed from captured HIP graphs
-    with the state in HBM, timed; a clone of the state taken at the start of
-    the timed window then runs the same steps again with HIP events around
-    every launch (on the library's stream), so the per-kernel durations
-    belong to exactly the timed steps.  Returns (timing dict, solver)."""
+def timed_window(case, opts, warmup, steps, tune=True):
+    """Launch geometry tuned (hydist_amd.tuning: from its cache, or timed on
+    clones over the first steps from t = 0), warm-up steps, then `steps`
+    steps replayed from captured HIP graphs with the state in HBM, timed; a
+    clone of the state taken at the start of the timed window then runs the
+    same steps again with HIP events around every launch (on the library's
+    stream), so the per-kernel durations belong to exactly the timed steps.
+    Returns (timing dict, solver)."""
+    from hydist_amd import tuning
     from hydist_amd._lib import Solver
     g = Solver(case, device=0)
     g.init()
+    launch = None
+    if tune:
+        launch = tuning.tune(g, 0.0, opts, steps=max(10, min(steps, 20)))
+        launch.pop("key", None)
     t = g.run(0.0, warmup, opts) if warmup else 0.0
     g.sync()
     shadow = g.clone()
@@ -317,7 +331,7 @@ def timed_window(case, opts, warmup, steps):
     shadow.profile(False)
     shadow.close()
     return {"value": wet_steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "wet_mean": wet_steps / steps,
-            "kt": kt, "t": t}, g
+            "kt": kt, "t": t, "launch": launch}, g
 
 
 def kernel_table(kt, wet_mean):
@@ -336,7 +350,7 @@ def river_line(a):
     from hydist_amd import synth
     case = synth.config(2)
     opts = case.opts()
-    w, g = timed_window(case, opts, a.warmup, a.steps)
+    w, g = timed_window(case, opts, a.warmup, a.steps, tune=not a.no_tune)
     g.close()
     roof = roofline(w["kt"], w["wet_mean"])
     kernels = {k: round(ms / n * 1e3, 2) for k, (ms, n) in w["kt"].items()}
@@ -363,7 +377,7 @@ def river_line(a):
     ens = None
     if not a.no_ensemble:
         try:
-            ens = ensemble_bench(a.ensemble_steps)
+            ens = ensemble_bench(a.ensemble_steps, tune=not a.no_tune)
         except Exception as e:  # reported, the river line still prints
             ens = {"error": f"{type(e).__name__}: {e}"}
 
@@ -382,7 +396,7 @@ def river_line(a):
         "data": "synthetic",
         "config": {"workload": case.name, "N": case.N, "M": case.M, "dx_m": 20.0, "dT_s": case.dT,
                    "mean_wet_cells": round(w["wet_mean"], 1), "physics": "hydrodynamics",
-                   "hbm_frac_step_algorithmic": w["value"] * B_STEP / HBM_PEAK},
+                   "hbm_frac_step_algorithmic": w["value"] * B_STEP / HBM_PEAK, "launch": w["launch"]},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_us": kernels,
@@ -400,7 +414,7 @@ def sediment_line(a):
     case.bed_change_start = 0.0
     case.bed_mode = BED_EVERY_STEP
     opts = case.opts()
-    w, g = timed_window(case, opts, a.warmup, a.steps)
+    w, g = timed_window(case, opts, a.warmup, a.steps, tune=not a.no_tune)
     g.close()
     return {"metric": "ADI wet-cell-steps/s with sediment and bed change (FP64)", "value": w["value"],
             "unit": "wet-cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -412,7 +426,7 @@ def sediment_line(a):
 
 
 def ensemble_line(a):
-    e = ensemble_bench(a.ensemble_steps, profile=True)
+    e = ensemble_bench(a.ensemble_steps, profile=True, tune=not a.no_tune)
     kt = e.pop("kt")
     return {"metric": "ensemble wet-cell-steps/s (FP64)", **e, "n_gpus": 1, "higher_is_better": True,
             "dtype": "f64", "data": "synthetic",

@@ -66,3 +66,38 @@ def test_ensemble_rejects_uninitialised_member():
     with pytest.raises(HydistError) as e:
         Ensemble([g])
     assert e.value.code == -6
+
+
+def test_tuned_launch_cached_and_state_unchanged(tmp_path, monkeypatch):
+    """hydist_amd.tuning: a case is timed once and then set from the cache;
+    tuning an ensemble leaves its members' states as they were, and the
+    tuned ensemble still gives each member's own bits."""
+    import numpy as np
+    from hydist_amd import tuning
+    monkeypatch.setenv("HYDIST_TUNE_CACHE", str(tmp_path / "launch.json"))
+    cases = small_members(True, seed=11)
+    op = cases[0].opts()
+    g = Solver(cases[0])
+    g.init()
+    first = tuning.tune(g, 0.0, op, steps=4)
+    again = tuning.tune(g, 0.0, op, steps=4)
+    assert not first["cached"] and again["cached"]
+    assert (again["rb_mult"], again["bw"]) == (first["rb_mult"], first["bw"])
+    assert first["rb_mult"] in tuning.RB_CHOICES and first["bw"] in tuning.BW_CHOICES
+    g.close()
+    alone = [Solver(c) for c in cases]
+    batch = [Solver(c) for c in cases]
+    for s in alone + batch:
+        s.init()
+    before = [s.get("z").copy() for s in batch]
+    info = tuning.tune_ensemble(batch, 0.0, op, steps=4)
+    assert len(info["trials"]) == len(tuning.RB_CHOICES) * len(tuning.BW_CHOICES)
+    for s, z in zip(batch, before):
+        assert np.array_equal(s.get("z"), z)
+    assert tuning.tune_ensemble(batch, 0.0, op, steps=4)["cached"]
+    for s in alone:
+        s.run(0.0, 20, op)
+    ens = Ensemble(batch)
+    ens.run(0.0, 20, op)
+    for k, (a, b) in enumerate(zip(alone, batch)):
+        assert_same_state(a, b, f"member {k} after tuning")
