@@ -126,6 +126,14 @@ __device__ __forceinline__ void aff_scan_step(double& al, double& be)
     al = al * a2;
 }
 
+/* v of lane `l` (wave-uniform l) through the scalar unit */
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 __device__ __forceinline__ bool same_bits(double x, double y)
 {
     return __double_as_longlong(x) == __double_as_longlong(y);
@@ -141,6 +149,25 @@ __device__ __forceinline__ void thomas_row(const double* sA, const double* sB, c
     double nbp = (D - (A * bp)) / e;
     ap = nap;
     bp = nbp;
+}
+
+/* bp of one forward row when its divisor e and y = RN(1/e) are known
+ * (Markstein's correction: q0 = RN(n y) is within an ulp of n / e, the
+ * remainder n - q0 e is exact by FMA, and RN(q0 + r y) is then RN(n / e)):
+ * the bits of (D - A bp) / e with a multiply and two FMAs instead of a
+ * division.  Outside the range where no intermediate underflows or
+ * overflows (|n| >= 2^-960, |q0| in [2^-1000, 2^1000]; e and y are checked
+ * by the caller) it divides. */
+__device__ __forceinline__ double bp_known_e(double A, double D, double bp, double e, double y)
+{
+    const double n = D - (A * bp);
+    const double q0 = n * y;
+    const double an = fabs(n), aq = fabs(q0);
+    if (an >= 0x1p-960 && an <= 0x1p1000 && aq >= 0x1p-1000 && aq <= 0x1p1000) {
+        const double r = fma(-q0, e, n);
+        return fma(r, y, q0);
+    }
+    return n / e;
 }
 
 /* Forward elimination of LDS rows g0 .. g1 (chunk-relative), entering with
@@ -221,22 +248,58 @@ __device__ __forceinline__ void spec_forward(const double* sA, const double* sB,
         double pa = __shfl_up(ca, 1, kSpecWave), pb = __shfl_up(cb, 1, kSpecWave);
         bool ok = exact || !act || (same_bits(wa, pa) && same_bits(wb, pb));
         unsigned long long bad = __ballot(!ok);
+        /* A lane that failed with its ap bits right (where the flow starts
+         * from rest every failure is of this kind: ap depends on A, B, C
+         * only and merges, bp decays ahead of the waves as fast as the
+         * estimate's error) has its rows' divisors e exactly; their
+         * reciprocals are taken here by every lane at once, so its repair
+         * below runs bp's chain without divisions (bp_known_e). */
+        double rA[kSpecMaxL], rD[kSpecMaxL], re[kSpecMaxL], ry[kSpecMaxL];
+        bool mk = false;
+        if (bad) {
+            mk = true;
+#pragma unroll
+            for (int q = 0; q < kSpecMaxL; ++q) {
+                rA[q] = 0.0; rD[q] = 0.0; re[q] = 1.0; ry[q] = 1.0;
+                if (sk + q < ek) {
+                    rA[q] = sA[sk + q];
+                    rD[q] = sD[sk + q];
+                    re[q] = rA[q] * (q == 0 ? wa : ra[q - 1]) + sB[sk + q];
+                    ry[q] = 1.0 / re[q];
+                    const double ae = fabs(re[q]);
+                    mk = mk && ae >= 0x1p-1000 && ae <= 0x1p1000;
+                }
+            }
+        }
         while (bad) {
             const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
-            double fa = __shfl(ca, f - 1, kSpecWave), fb = __shfl(cb, f - 1, kSpecWave);
+            double fa = readlane_f64(ca, f - 1), fb = readlane_f64(cb, f - 1);
             if (lane == f) {
-                ca = fa;
-                cb = fb;
+                if (mk && same_bits(wa, fa)) {
+                    /* ap of the lane's rows is exact already: bp only,
+                     * from registers */
+                    cb = fb;
 #pragma unroll
-                for (int q = 0; q < kSpecMaxL; ++q) {
-                    if (sk + q < ek) {
-                        thomas_row(sA, sB, sC, sD, sk + q, ca, cb);
-                        ra[q] = ca;
-                        rb[q] = cb;
+                    for (int q = 0; q < kSpecMaxL; ++q) {
+                        if (sk + q < ek) {
+                            cb = bp_known_e(rA[q], rD[q], cb, re[q], ry[q]);
+                            rb[q] = cb;
+                        }
+                    }
+                } else {
+                    ca = fa;
+                    cb = fb;
+#pragma unroll
+                    for (int q = 0; q < kSpecMaxL; ++q) {
+                        if (sk + q < ek) {
+                            thomas_row(sA, sB, sC, sD, sk + q, ca, cb);
+                            ra[q] = ca;
+                            rb[q] = cb;
+                        }
                     }
                 }
             }
-            double na = __shfl(ca, f, kSpecWave), nb = __shfl(cb, f, kSpecWave);
+            double na = readlane_f64(ca, f), nb = readlane_f64(cb, f);
             if (lane == f) ok = true;
             if (lane == f + 1) ok = exact || !act || (same_bits(wa, na) && same_bits(wb, nb));
             bad = __ballot(!ok);

@@ -53,6 +53,29 @@ def test_thomas_batch_bitwise():
         assert np.array_equal(x, oracle.thomas(*s))
 
 
+@pytest.mark.parametrize("cscale,margin", [(0.05, 0.05), (1.0, 2.0), (1.0, 100.0)])
+def test_thomas_batch_bitwise_decaying_rhs(cscale, margin):
+    """Right-hand sides that vanish after the first rows, as ahead of the
+    waves of a flow starting from rest: bp decays geometrically, the
+    speculative lanes' estimates never merge with it bit for bit, and almost
+    every lane is repaired with its divisors known (the reciprocal form of
+    the division: bp stays normal over all 1961 rows at the slowest decay
+    and reaches the underflow range, where that form divides instead,
+    within a few hundred rows at the others).  The result must still be
+    the serial solve's."""
+    rng = np.random.default_rng(1)
+    systems = []
+    for n in (200, 509, 1961):
+        A = rng.uniform(0.5, 1.0, size=n) * rng.choice([-1.0, 1.0], size=n)
+        C = cscale * rng.uniform(0.5, 1.0, size=n) * rng.choice([-1.0, 1.0], size=n)
+        B = (np.abs(A) + np.abs(C)) * (1.0 + margin) * rng.uniform(1.0, 1.05, size=n)
+        D = np.zeros(n)
+        D[:4] = rng.normal(size=4)
+        systems.append((A, B, C, D))
+    for x, s in zip(thomas_batch(systems), systems):
+        assert np.array_equal(x, oracle.thomas(*s))
+
+
 @pytest.mark.parametrize("make", [small_channel, small_river, small_river_sediment])
 def test_init_parity(make):
     case = make()
