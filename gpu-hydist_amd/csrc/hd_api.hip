@@ -1400,8 +1400,8 @@ int hd_ensemble_kernel_times(hd_ensemble* e, int n, const char** names, double* 
     return k;
 }
 
-int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const double* A, const double* B,
-                    const double* C, const double* D, double* x, int nrows)
+static int thomas_batch_impl(int device, int nsys, const int* off, const int* sn, const double* A, const double* B,
+                             const double* C, const double* D, double* x, int nrows, bool spec)
 {
     if (nsys < 0 || nrows < 0 || !off || !sn || !A || !B || !C || !D || !x) return fail(nullptr, HD_EINVAL, "bad args");
     for (int s = 0; s < nsys; ++s)
@@ -1428,13 +1428,26 @@ int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const d
         hipMemcpy(dC, C, rb, hipMemcpyHostToDevice);
         hipMemcpy(dD, D, rb, hipMemcpyHostToDevice);
         hipMemcpy(dx, x, rb, hipMemcpyHostToDevice);
-        hd_thomas_batch_device(nsys, doff, dsn, dA, dB, dC, dD, dx, dAp, dBp, nrows, 0);
+        if (spec) hd_thomas_spec_device(nsys, doff, dsn, dA, dB, dC, dD, dx, dAp, dBp, nrows, 0);
+        else hd_thomas_batch_device(nsys, doff, dsn, dA, dB, dC, dD, dx, dAp, dBp, nrows, 0);
         hipError_t e = hipDeviceSynchronize();
         if (e != hipSuccess) rc = fail(nullptr, HD_EHIP, hipGetErrorString(e));
         else hipMemcpy(x, dx, rb, hipMemcpyDeviceToHost);
     }
     for (void* p : bufs) hipFree(p);
     return rc;
+}
+
+int hd_thomas_batch(int device, int nsys, const int* off, const int* sn, const double* A, const double* B,
+                    const double* C, const double* D, double* x, int nrows)
+{
+    return thomas_batch_impl(device, nsys, off, sn, A, B, C, D, x, nrows, false);
+}
+
+int hd_thomas_batch_spec(int device, int nsys, const int* off, const int* sn, const double* A, const double* B,
+                         const double* C, const double* D, double* x, int nrows)
+{
+    return thomas_batch_impl(device, nsys, off, sn, A, B, C, D, x, nrows, true);
 }
 
 

@@ -160,5 +160,8 @@ void hd_launch_z_change(const HdBatch& b, const double* zref, double bound, doub
 int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
                            const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,
                            hipStream_t s);
+int hd_thomas_spec_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
+                           const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,
+                           hipStream_t s);
 
 #endif

@@ -1211,7 +1211,7 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
                     g0 = 1;
                 }
                 const int g1 = (re < sn - 1 ? re : sn - 1) - rb;
-                spec_forward(sA, sB, sC, sD, g0, g1, ap, bp);
+                spec_forward<true>(sA, sB, sC, sD, g0, g1, ap, bp);
                 if (c == 0 && sn > 0) {
                     __syncthreads();
                     if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }
@@ -2893,6 +2893,71 @@ __global__ void k_thomas_batch(int nsys, const int* off, const int* sn, const do
     for (int r = n - 1; r >= 0; --r) { xv = Bp[o + r] + (Ap[o + r] * xv); x[o + r] = xv; }
 }
 
+/* One wave per system: the speculative parallel Thomas of the run solvers
+ * (hd_thomas.h) on a caller's system, rows in LDS chunks of 510, the
+ * factors of earlier chunks through Ap / Bp (as k_adi_spec spills them).
+ * The same bits as k_thomas_batch; systems outside rows [0, nrows) are
+ * skipped the same way. */
+__global__ void __launch_bounds__(64) k_thomas_spec(int nsys, const int* off, const int* sn, const double* A,
+                                                    const double* B, const double* C, const double* D, double* x,
+                                                    double* Ap, double* Bp, int nrows)
+{
+    constexpr int RW = 510;
+    __shared__ double sA[RW], sB[RW], sC[RW], sD[RW];
+    const int s = blockIdx.x, lane = threadIdx.x;
+    if (s >= nsys) return;
+    const int o = off[s], n = sn[s];
+    if (n < 0 || o < 0 || n >= nrows - o) return;
+    if (n == 0) {
+        if (lane == 0) x[o] = D[o] / B[o];
+        return;
+    }
+    const int nch = n / RW + 1;
+    double ap = 0, bp = 0, xs = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int rb = RW * c;
+        const int re = rb + RW - 1 < n ? rb + RW - 1 : n;
+        for (int i = lane; i <= re - rb; i += kWave) {
+            sA[i] = A[o + rb + i]; sB[i] = B[o + rb + i]; sC[i] = C[o + rb + i]; sD[i] = D[o + rb + i];
+        }
+        __syncthreads();
+        int g0 = 0;
+        if (c == 0) {
+            ap = -sC[0] / sB[0];
+            bp = sD[0] / sB[0];
+            g0 = 1;
+        }
+        const int g1 = (re < n - 1 ? re : n - 1) - rb;
+        hd_spec::spec_forward<true>(sA, sB, sC, sD, g0, g1, ap, bp);
+        if (c == 0) {
+            __syncthreads();
+            if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }
+        }
+        if (re == n) xs = (sD[n - rb] - (sA[n - rb] * bp)) / (sB[n - rb] + (sA[n - rb] * ap));
+        __syncthreads();
+        if (c + 1 < nch) {
+            for (int i = lane; i <= re - rb; i += kWave) { Ap[o + rb + i] = sC[i]; Bp[o + rb + i] = sD[i]; }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) x[o + n] = xs;
+    double xc = xs;
+    for (int c = nch - 1; c >= 0; --c) {
+        const int rb = RW * c;
+        const int re = rb + RW - 1 < n ? rb + RW - 1 : n;
+        if (c + 1 < nch) {
+            __syncthreads();
+            for (int i = lane; i <= re - rb; i += kWave) { sC[i] = Ap[o + rb + i]; sD[i] = Bp[o + rb + i]; }
+            __syncthreads();
+        }
+        const int hi = (re < n - 1 ? re : n - 1) - rb;
+        xc = hd_spec::spec_backward(sC, sD, sA, hi, xc);
+        __syncthreads();
+        for (int i = lane; i <= hi; i += kWave) x[o + rb + i] = sA[i];
+        __syncthreads();
+    }
+}
+
 } // namespace
 
 /* ================================================================ launchers */
@@ -3079,6 +3144,15 @@ int hd_thomas_batch_device(int nsys, const int* off, const int* sn, const double
     if (nsys > 0)
         hipLaunchKernelGGL(k_thomas_batch, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap,
                            Bp, nrows);
+    return 0;
+}
+
+int hd_thomas_spec_device(int nsys, const int* off, const int* sn, const double* A, const double* B,
+                          const double* C, const double* D, double* x, double* Ap, double* Bp, int nrows,
+                          hipStream_t s)
+{
+    if (nsys > 0)
+        hipLaunchKernelGGL(k_thomas_spec, dim3(nsys), dim3(64), 0, s, nsys, off, sn, A, B, C, D, x, Ap, Bp, nrows);
     return 0;
 }
 

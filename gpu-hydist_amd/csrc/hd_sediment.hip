@@ -574,7 +574,7 @@ __global__ void __launch_bounds__(64) k_fs_spec(const HdDev* __restrict__ dv, do
                 g0 = 1;
             }
             const int g1 = ((re < sn - 1) ? re : sn - 1) - rb;
-            hd_spec::spec_forward(sA, sB, sC, sD, g0, g1, ap, bp);
+            hd_spec::spec_forward<false>(sA, sB, sC, sD, g0, g1, ap, bp);
             if (c == 0 && sn > 0) {
                 __syncthreads();
                 if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }

@@ -37,7 +37,16 @@ constexpr int kSpecWave = 64;
  * result equals the serial solve's bits in every case; the speculation only
  * shortens the critical path from the run's length to about kSpecWF + L rows
  * (L = rows per lane).  Back substitution, x_i = bp_i + ap_i x_{i+1}, is an
- * affine chain and is treated the same way with 2-entry affine maps. */
+ * affine chain and is treated the same way with 2-entry affine maps.
+ *
+ * Repairs (spec_forward<true>, the ADI sweeps): while a flow starts from
+ * rest, bp decays ahead of the waves as fast as the estimate's relative
+ * error and never merges, so long stretches of lanes fail with their ap
+ * already exact.  Then every row's divisor is known, and a failing lane
+ * redoes only its bp chain, from registers, with the division in
+ * Markstein's reciprocal form (bp_known_e) -- the same rounding as the
+ * division; spec_forward<false> (the FS sweeps, which need the registers
+ * for occupancy) keeps the plain repair. */
 constexpr int kSpecWF = 16;           /* forward warm-up rows after the estimate */
 constexpr int kSpecWB = 32;           /* back-substitution warm-up rows */
 constexpr int kSpecSerialF = 32;      /* up to this many rows the forward sweep stays on one lane */
@@ -155,19 +164,18 @@ __device__ __forceinline__ void thomas_row(const double* sA, const double* sB, c
  * (Markstein's correction: q0 = RN(n y) is within an ulp of n / e, the
  * remainder n - q0 e is exact by FMA, and RN(q0 + r y) is then RN(n / e)):
  * the bits of (D - A bp) / e with a multiply and two FMAs instead of a
- * division.  Outside the range where no intermediate underflows or
- * overflows (|n| >= 2^-960, |q0| in [2^-1000, 2^1000]; e and y are checked
- * by the caller) it divides. */
-__device__ __forceinline__ double bp_known_e(double A, double D, double bp, double e, double y)
+ * division, valid where no intermediate underflows or overflows (|n| >=
+ * 2^-960, |q0| in [2^-1000, 2^1000]; e and y are checked by the caller).
+ * Branch-free: `off` is set when a row leaves that range, and the caller
+ * then redoes the rows with divisions. */
+__device__ __forceinline__ double bp_known_e(double A, double D, double bp, double e, double y, bool& off)
 {
     const double n = D - (A * bp);
     const double q0 = n * y;
     const double an = fabs(n), aq = fabs(q0);
-    if (an >= 0x1p-960 && an <= 0x1p1000 && aq >= 0x1p-1000 && aq <= 0x1p1000) {
-        const double r = fma(-q0, e, n);
-        return fma(r, y, q0);
-    }
-    return n / e;
+    off = off || !(an >= 0x1p-960 && an <= 0x1p1000 && aq >= 0x1p-1000 && aq <= 0x1p1000);
+    const double r = fma(-q0, e, n);
+    return fma(r, y, q0);
 }
 
 /* Forward elimination of LDS rows g0 .. g1 (chunk-relative), entering with
@@ -176,7 +184,8 @@ __device__ __forceinline__ double bp_known_e(double A, double D, double bp, doub
  * read them: speculating lanes keep their own rows' factors in registers
  * until the sweep is verified).  At most 64 * kSpecMaxL rows; one wave,
  * uniform control; ends with a barrier. */
-__device__ __forceinline__ void spec_forward(const double* sA, const double* sB, double* sC, double* sD, int g0,
+template <bool kFastRepair>
+__device__ __forceinline__ void spec_forward(const double* sA, double* sB, double* sC, double* sD, int g0,
                                              int g1, double& ap, double& bp)
 {
     const int lane = threadIdx.x;
@@ -248,45 +257,80 @@ __device__ __forceinline__ void spec_forward(const double* sA, const double* sB,
         double pa = __shfl_up(ca, 1, kSpecWave), pb = __shfl_up(cb, 1, kSpecWave);
         bool ok = exact || !act || (same_bits(wa, pa) && same_bits(wb, pb));
         unsigned long long bad = __ballot(!ok);
-        /* A lane that failed with its ap bits right (where the flow starts
-         * from rest every failure is of this kind: ap depends on A, B, C
-         * only and merges, bp decays ahead of the waves as fast as the
-         * estimate's error) has its rows' divisors e exactly; their
-         * reciprocals are taken here by every lane at once, so its repair
-         * below runs bp's chain without divisions (bp_known_e). */
-        double rA[kSpecMaxL], rD[kSpecMaxL], re[kSpecMaxL], ry[kSpecMaxL];
-        bool mk = false;
-        if (bad) {
-            mk = true;
+        if constexpr (kFastRepair) {
+            /* Where the flow starts from rest every failure has the lane's
+             * ap bits right (ap depends on A, B, C only and merges; bp decays
+             * ahead of the waves as fast as the estimate's error), so the
+             * lane's rows' divisors e are exact: every lane takes A, D, e
+             * and 1/e of its rows into registers once a failure is seen, and
+             * the repair of such a lane runs bp's chain in the reciprocal
+             * form (bp_known_e) with no LDS access; the predecessor's state
+             * moves through the scalar unit. */
+            double rA[kSpecMaxL], rD[kSpecMaxL], re[kSpecMaxL], ry[kSpecMaxL];
+            bool mk = false;
+            if (bad) {
+                mk = true;
 #pragma unroll
-            for (int q = 0; q < kSpecMaxL; ++q) {
-                rA[q] = 0.0; rD[q] = 0.0; re[q] = 1.0; ry[q] = 1.0;
-                if (sk + q < ek) {
-                    rA[q] = sA[sk + q];
-                    rD[q] = sD[sk + q];
-                    re[q] = rA[q] * (q == 0 ? wa : ra[q - 1]) + sB[sk + q];
-                    ry[q] = 1.0 / re[q];
-                    const double ae = fabs(re[q]);
-                    mk = mk && ae >= 0x1p-1000 && ae <= 0x1p1000;
+                for (int q = 0; q < kSpecMaxL; ++q) {
+                    rA[q] = 0.0; rD[q] = 0.0; re[q] = 1.0; ry[q] = 1.0;
+                    if (sk + q < ek) {
+                        rA[q] = sA[sk + q];
+                        rD[q] = sD[sk + q];
+                        re[q] = rA[q] * (q == 0 ? wa : ra[q - 1]) + sB[sk + q];
+                        ry[q] = 1.0 / re[q];
+                        const double ae = fabs(re[q]);
+                        mk = mk && ae >= 0x1p-1000 && ae <= 0x1p1000;
+                    }
                 }
             }
-        }
-        while (bad) {
-            const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
-            double fa = readlane_f64(ca, f - 1), fb = readlane_f64(cb, f - 1);
-            if (lane == f) {
-                if (mk && same_bits(wa, fa)) {
-                    /* ap of the lane's rows is exact already: bp only,
-                     * from registers */
-                    cb = fb;
+            while (bad) {
+                const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
+                double fa = readlane_f64(ca, f - 1), fb = readlane_f64(cb, f - 1);
+                if (lane == f) {
+                    if (mk && same_bits(wa, fa)) {
+                        /* ap of the lane's rows is exact already: bp only */
+                        bool off = false;
+                        cb = fb;
 #pragma unroll
-                    for (int q = 0; q < kSpecMaxL; ++q) {
-                        if (sk + q < ek) {
-                            cb = bp_known_e(rA[q], rD[q], cb, re[q], ry[q]);
-                            rb[q] = cb;
+                        for (int q = 0; q < kSpecMaxL; ++q) {
+                            if (sk + q < ek) {
+                                cb = bp_known_e(rA[q], rD[q], cb, re[q], ry[q], off);
+                                rb[q] = cb;
+                            }
+                        }
+                        if (off) { /* a row outside the reciprocal form's range: divide */
+                            cb = fb;
+#pragma unroll
+                            for (int q = 0; q < kSpecMaxL; ++q) {
+                                if (sk + q < ek) {
+                                    cb = (rD[q] - (rA[q] * cb)) / re[q];
+                                    rb[q] = cb;
+                                }
+                            }
+                        }
+                    } else {
+                        ca = fa;
+                        cb = fb;
+#pragma unroll
+                        for (int q = 0; q < kSpecMaxL; ++q) {
+                            if (sk + q < ek) {
+                                thomas_row(sA, sB, sC, sD, sk + q, ca, cb);
+                                ra[q] = ca;
+                                rb[q] = cb;
+                            }
                         }
                     }
-                } else {
+                }
+                double na = readlane_f64(ca, f), nb = readlane_f64(cb, f);
+                if (lane == f) ok = true;
+                if (lane == f + 1) ok = exact || !act || (same_bits(wa, na) && same_bits(wb, nb));
+                bad = __ballot(!ok);
+            }
+        } else {
+            while (bad) {
+                const int f = __ffsll((long long)bad) - 1; /* >= 1: lane 0 starts exact */
+                double fa = __shfl(ca, f - 1, kSpecWave), fb = __shfl(cb, f - 1, kSpecWave);
+                if (lane == f) {
                     ca = fa;
                     cb = fb;
 #pragma unroll
@@ -298,11 +342,11 @@ __device__ __forceinline__ void spec_forward(const double* sA, const double* sB,
                         }
                     }
                 }
+                double na = __shfl(ca, f, kSpecWave), nb = __shfl(cb, f, kSpecWave);
+                if (lane == f) ok = true;
+                if (lane == f + 1) ok = exact || !act || (same_bits(wa, na) && same_bits(wb, nb));
+                bad = __ballot(!ok);
             }
-            double na = readlane_f64(ca, f), nb = readlane_f64(cb, f);
-            if (lane == f) ok = true;
-            if (lane == f + 1) ok = exact || !act || (same_bits(wa, na) && same_bits(wb, nb));
-            bad = __ballot(!ok);
         }
     }
     const int klast = (n - 1) / L;

@@ -22,6 +22,7 @@ EXPORTS = (
     "hd_field_info", "hd_upload", "hd_download", "hd_get_hmax", "hd_init", "hd_prepare", "hd_step", "hd_run",
     "hd_halfstep", "hd_bed_change", "hd_run_counted", "hd_reset_wet_counter", "hd_wet_cell_steps",
     "hd_wet_count", "hd_sync", "hd_struct_ptrs", "hd_profile", "hd_kernel_times", "hd_thomas_batch",
+    "hd_thomas_batch_spec",
     "hd_ensemble_create", "hd_ensemble_destroy", "hd_ensemble_last_error", "hd_ensemble_run",
     "hd_ensemble_profile", "hd_ensemble_kernel_times", "hd_flow_refresh", "hd_sediment_run",
     "hd_clone", "hd_autotune", "hd_set_launch", "hd_bed_snapshot", "hd_bed_extrapolate", "hd_copy_state",
@@ -79,6 +80,7 @@ def load() -> ctypes.CDLL:
         "hd_profile": (ci, [vp, ci]),
         "hd_kernel_times": (ci, [vp, ci, ctypes.POINTER(cp), dp, ip]),
         "hd_thomas_batch": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
+        "hd_thomas_batch_spec": (ci, [ci, ci, ip, ip, dp, dp, dp, dp, dp, ci]),
         "hd_thomas_batch_noalloc": (ci, [ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, vp]),
         "hd_k_Onetime_init": (ci, [vp]),
         "hd_k_Find_Calculation_limits": (ci, [vp]),
@@ -335,9 +337,10 @@ class Solver:
         return a.value, b.value
 
 
-def thomas_batch(systems, device=0):
+def thomas_batch(systems, device=0, spec=False):
     """Solve a list of (A, B, C, D) tridiagonal systems with the GPU Thomas
-    kernel; returns the list of solutions."""
+    kernel (spec: the run solvers' speculative parallel form,
+    hd_thomas_batch_spec); returns the list of solutions."""
     L = load()
     off, sn, rows = [], [], 0
     for A, *_ in systems:
@@ -349,8 +352,9 @@ def thomas_batch(systems, device=0):
     off_a = np.asarray(off, np.int32)
     sn_a = np.asarray(sn, np.int32)
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
-    check(L.hd_thomas_batch(device, len(systems), off_a.ctypes.data_as(ip), sn_a.ctypes.data_as(ip),
-                            *(c.ctypes.data_as(dp) for c in cat), x.ctypes.data_as(dp), rows))
+    fn = L.hd_thomas_batch_spec if spec else L.hd_thomas_batch
+    check(fn(device, len(systems), off_a.ctypes.data_as(ip), sn_a.ctypes.data_as(ip),
+             *(c.ctypes.data_as(dp) for c in cat), x.ctypes.data_as(dp), rows))
     return [x[o:o + n + 1] for o, n in zip(off, sn)]
 
 

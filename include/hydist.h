@@ -242,6 +242,14 @@ int hd_ensemble_kernel_times(hd_ensemble* e, int n, const char** names, double* 
 int hd_thomas_batch(int device, int nsys, const int* off, const int* sn,
                     const double* A, const double* B, const double* C,
                     const double* D, double* x, int nrows);
+/* The same systems through the run solvers' speculative parallel Thomas
+ * (one wave per system, rows in LDS chunks of 510; the tridiag of
+ * UVZSolver_multithread.cu:203-228 that k_adi_spec runs): the same bits as
+ * hd_thomas_batch, for unit tests of the speculation and its repairs on
+ * chosen systems. */
+int hd_thomas_batch_spec(int device, int nsys, const int* off, const int* sn,
+                         const double* A, const double* B, const double* C,
+                         const double* D, double* x, int nrows);
 
 /* Per-kernel entry points for unit parity.  Each runs the library's kernel(s)
  * for the reference kernel it is named after on the context's current state

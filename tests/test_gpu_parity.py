@@ -42,27 +42,28 @@ def pair(case):
     return o, g
 
 
-def test_thomas_batch_bitwise():
+@pytest.mark.parametrize("spec", [False, True])
+def test_thomas_batch_bitwise(spec):
     rng = np.random.default_rng(0)
     systems = []
-    for n in (1, 2, 3, 7, 64, 333, 1961):
+    for n in (1, 2, 3, 7, 33, 34, 64, 333, 510, 511, 1961):
         A = rng.normal(size=n); C = rng.normal(size=n)
         B = 3 + np.abs(A) + np.abs(C); D = rng.normal(size=n)
         systems.append((A, B, C, D))
-    for x, s in zip(thomas_batch(systems), systems):
+    for x, s in zip(thomas_batch(systems, spec=spec), systems):
         assert np.array_equal(x, oracle.thomas(*s))
 
 
 @pytest.mark.parametrize("cscale,margin", [(0.05, 0.05), (1.0, 2.0), (1.0, 100.0)])
-def test_thomas_batch_bitwise_decaying_rhs(cscale, margin):
+def test_thomas_spec_bitwise_decaying_rhs(cscale, margin):
     """Right-hand sides that vanish after the first rows, as ahead of the
     waves of a flow starting from rest: bp decays geometrically, the
     speculative lanes' estimates never merge with it bit for bit, and almost
-    every lane is repaired with its divisors known (the reciprocal form of
-    the division: bp stays normal over all 1961 rows at the slowest decay
-    and reaches the underflow range, where that form divides instead,
-    within a few hundred rows at the others).  The result must still be
-    the serial solve's."""
+    every lane of hd_thomas_batch_spec is repaired with its divisors known
+    (the reciprocal form of the division: bp stays normal over all 1961
+    rows at the slowest decay and reaches the underflow range, where that
+    form divides instead, within a few hundred rows at the others).  The
+    result must still be the serial solve's."""
     rng = np.random.default_rng(1)
     systems = []
     for n in (200, 509, 1961):
@@ -72,7 +73,7 @@ def test_thomas_batch_bitwise_decaying_rhs(cscale, margin):
         D = np.zeros(n)
         D[:4] = rng.normal(size=4)
         systems.append((A, B, C, D))
-    for x, s in zip(thomas_batch(systems), systems):
+    for x, s in zip(thomas_batch(systems, spec=True), systems):
         assert np.array_equal(x, oracle.thomas(*s))
 
 
