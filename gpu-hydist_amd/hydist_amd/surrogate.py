@@ -289,14 +289,16 @@ def bed_difference(d_ref: np.ndarray, d: np.ndarray, wet: np.ndarray) -> dict:
 def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 6, batch: int = 4,
              lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, check_every: int = 10,
              z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log_checks: bool = False,
-             log=print) -> dict:
+             extra_z_fracs=(), log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
     bed-change run with the plain one on the config-3 river (bed change every
     step, from hour 2): simulated hours per wall second of both, and the bed
     (dH) difference after one day and after `days` days.  Beside them, the
     FP64 method against itself: a twin run whose inflow series is scaled by
     1 + 1e-12, the smallest difference any other method could have, and the
-    check's statistics on plain steps."""
+    check's statistics on plain steps.  extra_z_fracs: further runs of the
+    morphological mode with the check's fraction bound set to each value
+    (the trade-off of speed against the bed's accuracy)."""
     warm = 2.0
     case = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0, SL=SEGMENT_LIMIT)
     case.bed_mode = BED_EVERY_STEP
@@ -357,14 +359,16 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
                     "dV_max": float(st[:, 2].max()), "samples": len(stats), "every_h": 6}
     log(f"  plain check statistics: {plain_checks}")
     modes = {}
-    for label, msub in (("full_sediment_steps", 0), (f"morph_{morph}", morph)):
+    runs = [("full_sediment_steps", 0, z_frac), (f"morph_{morph}", morph, z_frac)]
+    runs += [(f"morph_{morph}_z_frac_{zf:g}", morph, zf) for zf in extra_z_fracs]
+    for label, msub, zf in runs:
         g2 = fresh(case)
         t, wall, diffs, calls, acc, fb, adi, checks = tb, 0.0, [], 0, 0, 0, 0, 0
         stop = None
         for k, m in enumerate(marks):
             try:
                 sur = surrogate_run(case, t, m, g2, net, K, check_every=check_every, z_bound=z_bound,
-                                    z_frac=z_frac, mass_bound=mass_bound, morph_substeps=msub,
+                                    z_frac=zf, mass_bound=mass_bound, morph_substeps=msub,
                                     log=log if log_checks else None)
             except HydistError as e:
                 stop = f"stopped in the segment ending at {(m - tb) / 3600.0:.0f} h: {e}"
@@ -383,7 +387,8 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
             g2.close()
             continue
         rate = (marks[-1] - tb) / 3600.0 / wall
-        modes[label] = {"sim_hours_per_wall_s": rate, "speedup": rate / plain_rate, "surrogate_calls": calls,
+        modes[label] = {"z_frac": zf, "sim_hours_per_wall_s": rate, "speedup": rate / plain_rate,
+                        "surrogate_calls": calls,
                         "checks": checks, "accepted": acc, "fallbacks": fb, "adi_steps": adi,
                         "bed_1d": diffs[0], "bed_end": diffs[-1]}
         log(f"  surrogate ({label}): {wall:.1f} s, speed-up {rate / plain_rate:.2f}, {calls} calls, "
@@ -420,9 +425,11 @@ if __name__ == "__main__":
     ap.add_argument("--z-frac", type=float, default=1e-3)
     ap.add_argument("--mass-bound", type=float, default=1e-4)
     ap.add_argument("--log-checks", action="store_true")
+    ap.add_argument("--extra-z-fracs", default="", help="comma-separated further check bounds (morphological mode)")
     a = ap.parse_args()
     r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
                  days=a.days, layers=a.layers, morph=a.morph, check_every=a.check_every, z_bound=a.z_bound,
                  z_frac=a.z_frac, mass_bound=a.mass_bound, log_checks=a.log_checks,
+                 extra_z_fracs=tuple(float(x) for x in a.extra_z_fracs.split(",") if x),
                  log=lambda s: print(s, flush=True))
     print(json.dumps(r))
