@@ -336,11 +336,13 @@ def long_channel():
                                              ("run", small_river, 30), ("run", full_river, 60),
                                              ("run", long_channel, 60)])
 def test_solver_forms_identical(form, make, steps, monkeypatch):
-    """The solver forms (one wave per run with the speculative parallel
-    Thomas, one wave per run with the serial Thomas, one wave per line, rows
-    kernel + lane per line), hydrodynamics and sediment, give identical bits."""
+    """Each alternative solver form (one wave per run with the serial
+    Thomas, one wave per line, rows kernel + lane per line), hydrodynamics
+    and sediment, against the CPU checker, bit for bit (the default form,
+    one wave per run with the speculative parallel Thomas, is checked the
+    same way by the tests above)."""
     case = make()
-    ref = Solver(case)
+    ref = oracle.Oracle(case)
     monkeypatch.setenv("HYDIST_SOLVER", form)
     alt = Solver(case)
     monkeypatch.delenv("HYDIST_SOLVER")
@@ -349,7 +351,7 @@ def test_solver_forms_identical(form, make, steps, monkeypatch):
     op = case.opts()
     ref.run(0.0, steps, op)
     alt.run(0.0, steps, op)
-    assert_same_state(ref, alt, f"speculative vs {form}")
+    assert_same_state(ref, alt, f"checker vs {form}")
 
 
 @pytest.mark.parametrize("form,make,steps", [("window", small_river, 40), ("scan", small_river, 40),
@@ -358,10 +360,10 @@ def test_solver_forms_identical(form, make, steps, monkeypatch):
 def test_wetting_forms_identical(form, make, steps, monkeypatch):
     """The wetting pass's three ways -- candidates visited with their own
     loads (up to 8 per line), replayed from prefetched windows, or the whole
-    line scanned -- give identical bits (HYDIST_WETTING forces one way for
-    every line)."""
+    line scanned -- give the checker's bits (HYDIST_WETTING forces one way for
+    every line), each against the CPU checker, bit for bit."""
     case = make()
-    ref = Solver(case)
+    ref = oracle.Oracle(case)
     monkeypatch.setenv("HYDIST_WETTING", form)
     alt = Solver(case)
     monkeypatch.delenv("HYDIST_WETTING")
@@ -370,7 +372,7 @@ def test_wetting_forms_identical(form, make, steps, monkeypatch):
     op = case.opts()
     ref.run(0.0, steps, op)
     alt.run(0.0, steps, op)
-    assert_same_state(ref, alt, f"default wetting vs {form}")
+    assert_same_state(ref, alt, f"checker vs wetting {form}")
 
 
 def test_clone_copies_state_and_steps_identically():
