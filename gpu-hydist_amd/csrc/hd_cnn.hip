@@ -50,7 +50,13 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f3
 
 /* Y = conv3x3(X) (+ bias) with epilogue EPI; images [nimg][H][W][C]. */
 /* With a tile list (tl != nullptr) workgroup k covers tile tl[k] for k <
- * *tn and does nothing beyond; without one, workgroup k covers tile k. */
+ * *tn and does nothing beyond; without one, workgroup k covers tile k.
+ * The tile's input window (rows h0 - 1 .. h0 + 4, columns w0 - 1 .. w0 + 64,
+ * zero outside the image) is staged in LDS once, pixel pitch padded by 16
+ * bytes against bank conflicts, and the nine taps' A fragments are read
+ * from there; each tap's weights ([cout][cin]) pass through a double-buffered
+ * LDS slot, the next tap's loaded into registers while the current one is
+ * multiplied. */
 template <int CIN, int COUT, int EPI>
 __global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
                                               const float* __restrict__ bias, int nimg, int H, int W,
@@ -59,6 +65,13 @@ __global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ X, const 
                                               const int* __restrict__ tl, const int* __restrict__ tn)
 {
     constexpr int NB = COUT / 16;
+    constexpr int CP = CIN + 8;      /* padded channels per pixel in LDS */
+    constexpr int WR = 6, WC = 66;   /* window rows and columns */
+    constexpr int C8 = CIN / 8;      /* 16-byte chunks per pixel */
+    constexpr int TW = COUT * CIN;   /* weights of one tap */
+    constexpr int NW = (TW / 8 + 255) / 256;
+    __shared__ bf16 sx[WR * WC * CP];
+    __shared__ bf16 sw[2][TW];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int tilesW = (W + 63) >> 6, tilesH = (H + 3) >> 2;
     int t = blockIdx.x;
@@ -70,10 +83,21 @@ __global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ X, const 
     t /= tilesW;
     const int th = t % tilesH;
     const int b = t / tilesH;
-    const int h = th * 4 + wv;
-    if (b >= nimg || h >= H) return;
-    const int w0 = tw * 64;
+    if (b >= nimg) return;               /* uniform over the workgroup */
+    const int h0 = th * 4, w0 = tw * 64;
     const size_t img = (size_t)b * H * W;
+    for (int k = threadIdx.x; k < WR * WC * C8; k += 256) {
+        const int r = k / (WC * C8), rem = k - r * (WC * C8);
+        const int c = rem / C8, c8 = rem - c * C8;
+        const int hh = h0 - 1 + r, ww = w0 - 1 + c;
+        bf16x8 v = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W) v = *(const bf16x8*)(X + (img + (size_t)hh * W + ww) * CIN + c8 * 8);
+        *(bf16x8*)(sx + (r * WC + c) * CP + c8 * 8) = v;
+    }
+    for (int k = threadIdx.x; k < TW / 8; k += 256) *(bf16x8*)(sw[0] + k * 8) = *(const bf16x8*)(Wt + k * 8);
+    __syncthreads();
+    const int h = h0 + wv;
+    const bool live = h < H;   /* every wave stays for the barriers */
     f32x4 acc[4][NB];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
@@ -81,32 +105,50 @@ __global__ void __launch_bounds__(256) k_conv(const bf16* __restrict__ X, const 
         for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int kq = 8 * (l >> 4);
     for (int tap = 0; tap < 9; ++tap) {
-        const int hh = h + tap / 3 - 1, dx = tap % 3 - 1;
-        const bool rowok = hh >= 0 && hh < H;
+        const int r = wv + tap / 3, dx = tap % 3 - 1;
+        const bf16* swt = sw[tap & 1];
+        bf16x8 nw[NW];
+        if (tap < 8) {
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const int k = threadIdx.x + 256 * q;
+                if (k < TW / 8) nw[q] = *(const bf16x8*)(Wt + (size_t)(tap + 1) * TW + k * 8);
+            }
+        }
 #pragma unroll
         for (int c0 = 0; c0 < CIN; c0 += 32) {
             bf16x8 bf[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = nb * 16 + (l & 15);
-                bf[nb] = *(const bf16x8*)(Wt + ((size_t)(tap * COUT + co) * CIN + c0 + kq));
+                bf[nb] = *(const bf16x8*)(swt + co * CIN + c0 + kq);
             }
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) {
-                const int ww = w0 + mb * 16 + (l & 15) + dx;
-                bf16x8 af;
-                if (rowok && ww >= 0 && ww < W) af = *(const bf16x8*)(X + (img + (size_t)hh * W + ww) * CIN + c0 + kq);
-                else af = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                const int c = mb * 16 + (l & 15) + dx + 1;
+                const bf16x8 af = *(const bf16x8*)(sx + (r * WC + c) * CP + c0 + kq);
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma(af, bf[nb], acc[mb][nb]);
             }
         }
+        if (tap < 8) {
+            /* the other slot was last read in the previous tap, which every
+             * wave finished before the barrier that ended it */
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const int k = threadIdx.x + 256 * q;
+                if (k < TW / 8) *(bf16x8*)(sw[(tap + 1) & 1] + k * 8) = nw[q];
+            }
+            __syncthreads();
+        }
     }
+    if (!live) return;
+    const int w0e = w0;
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int w = w0 + mb * 16 + 4 * (l >> 4) + r;
+            const int w = w0e + mb * 16 + 4 * (l >> 4) + r;
             if (w >= W) continue;
             const size_t p = img + (size_t)h * W + w;
 #pragma unroll
