@@ -265,6 +265,45 @@ def surrogate_run(case, t0: float, t1: float, g: Solver, net: cnn.Network, K: in
     return rep
 
 
+def perfect_flow_run(case, t0: float, t1: float, g: Solver, K: int, morph_substeps: int = 0) -> RunReport:
+    """The accelerated scheme with a perfect network: each macro step takes
+    the true FP64 flow at t + K dT (a clone of the state stepped with plain
+    ADI; the whole state but the concentration and the bed, FS / tFS / dH)
+    in place of the prediction, then runs the sediment half on that frozen
+    flow as surrogate_run does.  Slower than plain ADI (it runs it); the
+    bed's departure from a plain run measures the scheme alone."""
+    op = case.opts()
+    rep = RunReport(0, 0, 0)
+    t = t0
+    g.sync()
+    w0 = time.perf_counter()
+    while t < t1 - 1e-9:
+        k = min(K, int(round((t1 - t) / case.dT)))
+        rep.macro_steps += 1
+        sed = {f: g.get(f) for f in ("FS", "tFS", "dH")}
+        truth = g.clone()
+        truth.run(t, k, op)
+        g.copy_state_from(truth)
+        truth.close()
+        for f, a in sed.items():
+            g.set(f, a)
+        if morph_substeps > 0 and k == K:
+            ts = t
+            if morph_substeps > 1:
+                t = g.sediment_run(t, morph_substeps - 1, op)
+            g.bed_snapshot()
+            g.sediment_run(t, 1, op)
+            g.bed_extrapolate(K - morph_substeps)
+            t = ts + K * case.dT
+        else:
+            t = g.sediment_run(t, k, op)
+    g.sync()
+    rep.wall_s = time.perf_counter() - w0
+    rep.sim_hours = (t - t0) / 3600.0
+    rep.sim_hours_per_wall_s = rep.sim_hours / rep.wall_s
+    return rep
+
+
 def check_statistics(case, g: Solver, t: float, z_bound: float = 5e-3):
     """What the surrogate's check measures, for one plain FP64 step from the
     current state (left unchanged): (max |dz|, fraction of wet cells over
@@ -289,7 +328,7 @@ def bed_difference(d_ref: np.ndarray, d: np.ndarray, wet: np.ndarray) -> dict:
 def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 60, epochs: int = 6, batch: int = 4,
              lr: float = 5e-4, days: float = 7.0, layers: int = 8, morph: int = 6, check_every: int = 10,
              z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log_checks: bool = False,
-             extra_z_fracs=(), log=print) -> dict:
+             extra_z_fracs=(), perfect_flow: bool = False, log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
     bed-change run with the plain one on the config-3 river (bed change every
     step, from hour 2): simulated hours per wall second of both, and the bed
@@ -298,7 +337,10 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
     1 + 1e-12, the smallest difference any other method could have, and the
     check's statistics on plain steps.  extra_z_fracs: further runs of the
     morphological mode with the check's fraction bound set to each value
-    (the trade-off of speed against the bed's accuracy)."""
+    (the trade-off of speed against the bed's accuracy).  perfect_flow: also
+    the scheme with the true flow in place of the network (perfect_flow_run)
+    over the first day, with the sediment every step and with the
+    morphological factor."""
     warm = 2.0
     case = synth.config(3, scale=scale, hours=warm + days * 24.0 + 1.0, SL=SEGMENT_LIMIT)
     case.bed_mode = BED_EVERY_STEP
@@ -394,9 +436,18 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
         log(f"  surrogate ({label}): {wall:.1f} s, speed-up {rate / plain_rate:.2f}, {calls} calls, "
             f"{acc}/{checks} checks passed, bed at the end {diffs[-1]}")
         g2.close()
+    perfect = {}
+    if perfect_flow:
+        for label, msub in (("sediment_every_step", 0), (f"morph_{morph}", morph)):
+            g4 = fresh(case)
+            perfect_flow_run(case, tb, marks[0], g4, K, morph_substeps=msub)
+            perfect[label] = bed_difference(plain_beds[0][0], g4.get("dH"), plain_beds[0][1])
+            log(f"  perfect flow ({label}): bed after 1 day {perfect[label]}")
+            g4.close()
     g1.close()
     net.close()
     return {
+        "perfect_flow_bed_1d": perfect,
         "workload": f"{case.name} bed change {days:g} d (every step), K={K}",
         "plain_sim_hours_per_wall_s": plain_rate, "plain_check_statistics": plain_checks,
         "twin_bed_1d": twin_diff[0], "twin_bed_end": twin_diff[-1], "twin_stopped": twin_stop, "modes": modes,
@@ -426,10 +477,12 @@ if __name__ == "__main__":
     ap.add_argument("--mass-bound", type=float, default=1e-4)
     ap.add_argument("--log-checks", action="store_true")
     ap.add_argument("--extra-z-fracs", default="", help="comma-separated further check bounds (morphological mode)")
+    ap.add_argument("--perfect-flow", action="store_true", help="also the scheme with the true flow (first day)")
     a = ap.parse_args()
     r = evaluate(scale=a.scale, variants=a.variants, pairs=a.pairs, K=a.K, epochs=a.epochs, batch=a.batch,
                  days=a.days, layers=a.layers, morph=a.morph, check_every=a.check_every, z_bound=a.z_bound,
                  z_frac=a.z_frac, mass_bound=a.mass_bound, log_checks=a.log_checks,
                  extra_z_fracs=tuple(float(x) for x in a.extra_z_fracs.split(",") if x),
+                 perfect_flow=a.perfect_flow,
                  log=lambda s: print(s, flush=True))
     print(json.dumps(r))

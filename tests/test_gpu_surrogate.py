@@ -92,3 +92,27 @@ def test_surrogate_accepted_at_real_bounds():
     d = S.bed_difference(plain.get("dH"), sur.get("dH"), wet)
     assert d["p99_m"] <= 0.015 and d["frac_within_1mm"] >= 0.9, d
     assert np.isfinite(sur.get("z")[wet]).all()
+
+
+def test_perfect_flow_scheme():
+    """The scheme with the true flow in place of the network: the flow does
+    not depend on the bed (dH is never fed back into the depths), so u, v, z
+    equal a plain run's bit for bit after every macro step, and the bed run
+    on the flow frozen over each macro step stays close to the plain run's."""
+    case = synth.config(3, scale=SCALE, hours=6.0, SL=S.SEGMENT_LIMIT)
+    case.bed_mode = BED_EVERY_STEP
+    op = case.opts()
+    plain = Solver(case)
+    plain.init()
+    tb = 2.0 * 3600.0
+    plain.run(0.0, int(round(tb / case.dT)), op)
+    pf = plain.clone()
+    S.plain_run(case, tb, tb + 3600.0, plain)
+    rep = S.perfect_flow_run(case, tb, tb + 3600.0, pf, 10)
+    assert rep.macro_steps == 72 and abs(rep.sim_hours - 1.0) < 1e-9
+    for f in ("u", "v", "z", "khouot"):
+        assert np.array_equal(plain.get(f), pf.get(f)), f
+    wet = plain.get("khouot") == 0
+    d = S.bed_difference(plain.get("dH"), pf.get("dH"), wet)
+    assert np.isfinite(pf.get("dH")[wet]).all()
+    assert d["p99_m"] <= 0.005, d      # measured 1.7 mm
