@@ -594,16 +594,21 @@ static int check_series(hd_ctx* c, double t, const hd_step_opts& o)
 }
 
 /* One half-step of Engine.py:146-239 (which = 0, UZ) or :260-361 (which = 1,
- * VZ), enqueued on the stream for every context of the batch.  The device
- * time is advanced by the boundary kernel.  Returns in *slo / *eoff the row
- * range of a following bed change (rows slo .. N - eoff - 1). */
+ * VZ), enqueued on the stream for every context of the batch.  The boundary
+ * values of the half-step (Update_Boundary_Value) and the device time step:
+ * in the default form with the boundary series (not the analytic channel
+ * tide, not the narrow-channel index shifts), the sweep's runs that own the
+ * edge cells write the values and the explicit pass after it advances the
+ * time; otherwise a k_boundary launch does both.  Returns in *slo / *eoff
+ * the row range of a following bed change (rows slo .. N - eoff - 1). */
 static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sediment, int count_wet, int ntiles,
                          int sed_ntiles, int* slo_out, int* eoff_out)
 {
     hipStream_t s = x.stream;
     bool channel = o.kenhhepd || o.kenhhepng;
     int an = analytic_mode(o) ? 1 : 0;
-    launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, s); });
+    const int own = (x.solver == 0 && !an && !channel) ? 1 : 0;
+    if (!own) launch(x, "boundary", [&] { hd_launch_boundary_t(x.b(), an, s); });
     int slo = 2, eoff = 0;
     if (which == 0) {
         int shift = (channel && o.kenhhepd == 1) ? 1 : 0;
@@ -615,12 +620,12 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else if (x.solver == 3) {
             launch(x, "adi_run_uz", [&] { hd_launch_adi_run(x.b(), 1, shift, x.run_blocks_u, s); });
         } else {
-            launch(x, "adi_spec_uz", [&] { hd_launch_adi_spec(x.b(), 1, shift, x.run_blocks_u, s); });
+            launch(x, "adi_spec_uz", [&] { hd_launch_adi_spec(x.b(), 1, shift, x.run_blocks_u, own, s); });
         }
         if (x.solver == 0) {
             /* smoothing of t_u, the cells of solveV and the rows' margins by
              * row tiles in one launch */
-            launch(x, "smooth_u_explicit_v", [&] { hd_launch_smooth_u_explicit_v(x.b(), x.bw, ntiles, s); });
+            launch(x, "smooth_u_explicit_v", [&] { hd_launch_smooth_u_explicit_v(x.b(), x.bw, ntiles, own, s); });
             x.swap_u();
         } else {
             launch(x, "smooth_u", [&] { hd_launch_normalize_post(x.b(), 1, 0, 1, x.bw, ntiles, s); });
@@ -652,12 +657,12 @@ static void enqueue_half(Exec& x, int which, const hd_step_opts& o, bool sedimen
         } else if (x.solver == 3) {
             launch(x, "adi_run_vz", [&] { hd_launch_adi_run(x.b(), 0, shift, x.run_blocks_v, s); });
         } else {
-            launch(x, "adi_spec_vz", [&] { hd_launch_adi_spec(x.b(), 0, shift, x.run_blocks_v, s); });
+            launch(x, "adi_spec_vz", [&] { hd_launch_adi_spec(x.b(), 0, shift, x.run_blocks_v, own, s); });
         }
         if (x.solver == 0) {
             /* smoothing of t_v, the cells of solveU and the columns' margins
              * by row tiles in one launch */
-            launch(x, "smooth_v_explicit_u", [&] { hd_launch_smooth_v_explicit_u(x.b(), x.bw, ntiles, s); });
+            launch(x, "smooth_v_explicit_u", [&] { hd_launch_smooth_v_explicit_u(x.b(), x.bw, ntiles, own, s); });
             x.swap_v();
         } else {
             launch(x, "smooth_v", [&] { hd_launch_normalize_post(x.b(), 0, 0, 0, x.bw, ntiles, s); });
@@ -1533,7 +1538,7 @@ int hd_k_Update_Boundary_Value(hd_ctx* c, double t, int analytic)
 static int k_sweep(hd_ctx* c, int isU, int shift)
 {
     Exec x; int rc = unit_begin(c, x); if (rc) return rc;
-    hd_launch_adi_spec(x.b(), isU, shift, isU ? x.run_blocks_u : x.run_blocks_v, x.stream);
+    hd_launch_adi_spec(x.b(), isU, shift, isU ? x.run_blocks_u : x.run_blocks_v, 0, x.stream);
     return unit_end(c, false);
 }
 

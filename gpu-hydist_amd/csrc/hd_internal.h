@@ -90,7 +90,8 @@ __device__ __forceinline__ const HdDevG& dev_view(const HdDev* dv)
 }
 
 /* The entries of a packed run table (cnt of them) that this workgroup of a
- * one-wave-per-run solver takes: start, end and stride of its loop.
+ * one-wave-per-run solver takes, the table being dealt to the first G
+ * workgroups of the launch: start, end and stride of its loop.
  * Workgroups are dealt to the eight XCDs in turn (workgroup b to XCD b % 8)
  * and neighbouring table entries are neighbouring lines (each block of the
  * table scan reserves its lines' entries together), so XCD x takes the x-th
@@ -99,9 +100,8 @@ __device__ __forceinline__ const HdDevG& dev_view(const HdDev* dv)
 struct RunSlice {
     int start, end, step;
 };
-__device__ __forceinline__ RunSlice run_slice(int cnt)
+__device__ __forceinline__ RunSlice run_slice(int cnt, int G = (int)gridDim.x)
 {
-    const int G = (int)gridDim.x;
     const int nx = G < 8 ? 1 : 8;
     const int x = (int)blockIdx.x % nx, slot = (int)blockIdx.x / nx;
     const int per = (G - x + nx - 1) / nx; /* workgroups on XCD x */
@@ -131,14 +131,16 @@ void hd_launch_adi_coef(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_line(const HdBatch& b, int isU, int shift, hipStream_t s);
 void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
-void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s);
+/* bnd: the runs write the half-step's boundary values (series mode) */
+void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, int bnd, hipStream_t s);
 void hd_launch_normalize_post(const HdBatch& b, int isU, int post, int horizontal, int bw, int ntiles, hipStream_t s);
 void hd_launch_explicit(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_margins(const HdBatch& b, int isU, hipStream_t s);
 void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_t s);
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
-void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s);
-void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s);
+/* adv: advance the device time by dT/2 (after a sweep that wrote the boundary) */
+void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, int adv, hipStream_t s);
+void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, int adv, hipStream_t s);
 /* mode: 0 the default wetting, 1 candidate windows for every line, 2 whole-line scans (cross-checks) */
 void hd_launch_wetting(const HdBatch& b, int horizontal, int mode, hipStream_t s);
 void hd_launch_smooth_wet(const HdBatch& b, int isU, int horizontal, int ntiles, int mode, hipStream_t s);

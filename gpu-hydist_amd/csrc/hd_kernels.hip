@@ -249,6 +249,175 @@ __global__ void k_boundary(const HdDev* __restrict__ dv, int analytic)
     }
 }
 
+/* The device time step of k_boundary (t = tdev + dT/2, with its series
+ * range flag) for a half-step whose boundary values the sweep's runs
+ * wrote: one thread, in the launch after the sweep. */
+__device__ __forceinline__ void advance_time(const HdDevG& d)
+{
+    const double t = d.tdev[0] + d.P.dT * 0.5;
+    d.tdev[0] = t;
+    const int t1 = (int)(t / 3600);
+    if (t1 < 0 || t1 + 1 >= d.total_time || t1 + 1 >= d.series_len) d.flags[1] = 1;
+}
+
+/* The first wet run of each edge line (column M, column 2, row 2, row N):
+ * its first cell a and its cells [lo, hi] clipped to [2, max(M, N)] as
+ * k_boundary's loop (hi < lo when the line is dry).  The same for every run
+ * of a sweep, so loaded once per workgroup. */
+struct EdgeRuns {
+    int lo[4], hi[4], a[4];
+};
+
+__device__ __forceinline__ EdgeRuns edge_runs(const HdDevG& d)
+{
+    const int N = d.N, M = d.M, SL = d.SL;
+    const int imax = M > N ? M : N;
+    EdgeRuns r;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool col = e < 2;
+        const int loc = e == 0 ? M : (e == 3 ? N : 2);
+        const int mc = col ? d.mocj[loc] : d.moci[loc];
+        const int a = col ? d.dauj[loc * SL] : d.daui[loc * SL];
+        const int b = col ? d.cuoij[loc * SL] : d.cuoii[loc * SL];
+        r.a[e] = a;
+        r.lo[e] = a > 2 ? a : 2;
+        r.hi[e] = mc > 0 ? (b < imax ? b : imax) : r.lo[e] - 1;
+    }
+    return r;
+}
+
+/* One boundary value of the series mode (Boundary_value + FS_boundary,
+ * support_funcs.cu:544-617, as k_boundary) of edge e (0 up: column M, 1
+ * down: column 2, 2 left: row 2, 3 right: row N) at its index i (the row of
+ * a column edge, the column of a row edge), at time t1 + t2 hours. */
+__device__ __forceinline__ void edge_cell(const HdDevG& d, int e, int i, int t1, double t2)
+{
+    const int N = d.N, M = d.M, W = d.W, SL = d.SL;
+    const bool col = e < 2;
+    const int loc = e == 0 ? M : (e == 3 ? N : 2);
+    const int ext = e == 0 ? M + 1 : (e == 3 ? N + 1 : 1);
+    const int fi = col ? i : loc, fj = col ? loc : i;
+    const double* bc = e == 0 ? d.bc_up : e == 1 ? d.bc_down : e == 2 ? d.bc_left : d.bc_right;
+    const double* CCs = e == 0 ? d.CC_u : e == 1 ? d.CC_d : e == 2 ? d.CC_l : d.CC_r;
+    const double* he = d.hi + (e == 0 ? 0 : e == 1 ? (N + 3) : e == 2 ? 2 * (N + 3) : 2 * (N + 3) + (M + 3));
+    double* barr = e == 0 ? d.vbt : e == 1 ? d.vbd : e == 2 ? d.ubt : d.ubp;
+    const int bt0 = d.boundary_type[e * SL];
+    const double bv = bc[t1] * (1.0 - t2) + bc[t1 + 1] * t2;
+    if (bt0 == 2) {
+        barr[i] = bv * he[i];
+        const double cv = CCs[t1] * (1.0 - t2) + CCs[t1 + 1] * t2;
+        d.FS[fi * W + fj] = cv * d.htaiz_bd[fi * W + fj] * d.hmax[e];
+    } else if (col) {
+        d.t_z[i * W + loc] = bv;
+        d.t_z[i * W + ext] = bv;
+    } else {
+        d.t_z[loc * W + i] = bv;
+        d.t_z[ext * W + i] = bv;
+    }
+}
+
+/* The half-step time of the boundary series: t = tdev + dT/2 as hours
+ * t1 + t2; false when the series is too short (advance_time flags it). */
+__device__ __forceinline__ bool boundary_time(const HdDevG& d, int& t1, double& t2)
+{
+    const double t = d.tdev[0] + d.P.dT * 0.5;
+    t1 = (int)(t / 3600);
+    t2 = (t - (3600.0 * t1)) / 3600.0;
+    return !(t1 < 0 || t1 + 1 >= d.total_time || t1 + 1 >= d.series_len);
+}
+
+/* The boundary values of the series mode that one wet run of a sweep owns,
+ * written by the run itself before it reads its end values, in place of a
+ * k_boundary launch before the sweep.  Edge e's cells are those of the
+ * first wet run of its edge line.  An edge along the sweep's lines (UZ:
+ * columns M and 2; VZ: rows 2 and N) is owned by that line's first run,
+ * which holds all of its cells.  A cell of an edge across the lines is
+ * owned by the run of its line that starts (row or column 2) or ends (row N,
+ * column M) there, when the line has one: the run rule ends a run at N or M
+ * only when the two cells before are wet, so some edge cells lie in no run:
+ * across_boundary writes those.  A run reads only its own
+ * line's boundary values (t_z at its first / last cell and its line's
+ * discharges, used only when the run starts at 2 or ends at N / M), so no
+ * run reads a value that another run writes.  The edges are written in the
+ * reference's order (0 up, 1 down, 2 left, 3 right) by the lane of the
+ * cell, so at a corner the later edge's value stays, as k_boundary's yield
+ * rule gives.  The device time is read here and advanced by the next launch
+ * (k_smooth_*_explicit_*). */
+template <bool isU>
+__device__ __forceinline__ void run_boundary(const HdDevG& d, const EdgeRuns& er, int line, int first, int last,
+                                             int lane)
+{
+    const int N = d.N, M = d.M;
+    /* which cells k of this run each edge writes: [klo, khi] */
+    int klo[4], khi[4];
+    bool any = false, along = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const bool col = e < 2;
+        const int loc = e == 0 ? M : (e == 3 ? N : 2);
+        if (isU == col) {
+            const bool mine = line == loc && first == er.a[e] && er.hi[e] >= er.lo[e];
+            klo[e] = er.lo[e];
+            khi[e] = mine ? er.hi[e] : er.lo[e] - 1;
+            along = along || mine;
+        } else {
+            const bool mine = line >= er.lo[e] && line <= er.hi[e] && (loc == 2 ? first == 2 : last == loc);
+            klo[e] = loc;
+            khi[e] = mine ? loc : loc - 1;
+        }
+        any = any || khi[e] >= klo[e];
+    }
+    if (!any) return;
+    int t1;
+    double t2;
+    if (!boundary_time(d, t1, t2)) return;
+    auto cell = [&](int k) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (k >= klo[e] && k <= khi[e]) edge_cell(d, e, (isU == (e < 2)) ? k : line, t1, t2);
+    };
+    if (along) {
+        for (int k = first + lane; k <= last; k += kWave) cell(k);
+    } else if (lane == 0) {
+        cell(first);
+        if (last != first) cell(last);
+    }
+}
+
+/* The cells of the two edges across a sweep's lines (UZ: rows 2 and N; VZ:
+ * columns M and 2) that no run of the sweep holds, by one extra workgroup
+ * of the sweep.  A cell is held when its line has a run that starts (at
+ * row or column 2) or ends (at row N, column M) there: that run writes it
+ * in run_boundary and may overwrite it with its solution afterwards, so it
+ * is left alone here.  Runs are sorted along their line, so the line's
+ * first table entry tells whether one starts at 2 and its last whether one
+ * ends at N / M.  The other cells are read by no run of the sweep, only by
+ * the launches after it, and no other edge writes them: a corner that a
+ * row edge also covers lies in that row's first run, which then ends at M
+ * or starts at 2 and holds it. */
+template <bool isU>
+__device__ __forceinline__ void across_boundary(const HdDevG& d, const EdgeRuns& er, int lane)
+{
+    const int N = d.N, M = d.M, SL = d.SL;
+    int t1;
+    double t2;
+    if (!boundary_time(d, t1, t2)) return;
+    /* the crossing lines' tables: columns for UZ, rows for VZ */
+    const int* moc = isU ? d.mocj : d.moci;
+    const int* dau = isU ? d.dauj : d.daui;
+    const int* cuoi = isU ? d.cuoij : d.cuoii;
+#pragma unroll
+    for (int e = isU ? 2 : 0; e < (isU ? 4 : 2); ++e) {
+        const int loc = e == 0 ? M : (e == 3 ? N : 2);
+        for (int i = er.lo[e] + lane; i <= er.hi[e]; i += kWave) {
+            const int mc = moc[i] < SL ? moc[i] : SL;
+            const bool held = mc > 0 && (loc == 2 ? dau[i * SL] == 2 : cuoi[i * SL + mc - 1] == loc);
+            if (!held) edge_cell(d, e, i, t1, t2);
+        }
+    }
+}
+
 /* ================================================================ ADI rows */
 
 struct Pre {
@@ -1095,7 +1264,7 @@ __device__ __forceinline__ Pre shfl_pre(const Pre& a, int src)
  * together, so a short run costs one memory round trip after its table
  * entry. */
 template <bool isU>
-__global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv, int shift)
+__global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv, int shift, int bnd)
 {
     const HdDevG& d = dev_view(dv);
     const int sidx = 2 + shift, eidx = isU ? d.M - shift : d.N;
@@ -1119,7 +1288,16 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
     double* tq = isU ? d.t_u : d.t_v;
     const double* hh = d.h;
     const int nbr = (lane + 1) & (kWave - 1);
-    const RunSlice sl = run_slice(cnt);
+    EdgeRuns er{};
+    if (bnd) {
+        er = edge_runs(d);
+        /* the extra last workgroup: the across edges' cells */
+        if ((int)blockIdx.x == (int)gridDim.x - 1) {
+            across_boundary<isU>(d, er, lane);
+            return;
+        }
+    }
+    const RunSlice sl = run_slice(cnt, (int)gridDim.x - bnd);
     for (int ri = sl.start; ri < sl.end; ri += sl.step) {
         const int4 rn = runs[ri];
         const int line = rn.x, first = rn.y, last = rn.z;
@@ -1135,6 +1313,9 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
             hb0 = hh[line * W + first - 1]; hb1 = hh[(line - 1) * W + first - 1];
             he0 = hh[line * W + last]; he1 = hh[(line - 1) * W + last];
         }
+        /* this run's boundary cells, before its end values are read (one
+         * wave: its own stores are seen by its later loads) */
+        if (bnd) run_boundary<isU>(d, er, line, first, last, lane);
         const double TZ_l = tz[at(first)];
         const double TZ_r = tz[at(last)];
         const double ub1 = isU ? d.ubt[line] : d.vbd[line];
@@ -1934,9 +2115,10 @@ __device__ __forceinline__ void dry_store(const HdDevG& d, int i, int j, int hor
  * unsmoothed buffer, the margins of the rows' runs (margin_at), and the
  * drying flags, wet bits and wetting candidates of the row reset
  * (dry_load / dry_store). */
-__global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled)
+__global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled, int adv)
 {
     const HdDevG& d = dev_view(dv);
+    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) advance_time(d);
     int i, j;
     if (!row_cell(d, tiled, i, j)) return;
     const int N = d.N, M = d.M, W = d.W;
@@ -1984,9 +2166,10 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
  * boundary between them; the margins of the columns' runs (margin_at) and
  * the drying flags, wet bits and wetting candidates of the column reset
  * too. */
-__global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled)
+__global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled, int adv)
 {
     const HdDevG& d = dev_view(dv);
+    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) advance_time(d);
     int i, j;
     if (!row_cell(d, tiled, i, j)) return;
     const int N = d.N, M = d.M, W = d.W;
@@ -3045,10 +3228,11 @@ void hd_launch_adi_run(const HdBatch& b, int isU, int shift, int nblocks, hipStr
     else hipLaunchKernelGGL(k_adi_run<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
 }
 
-void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, hipStream_t s)
+void hd_launch_adi_spec(const HdBatch& b, int isU, int shift, int nblocks, int bnd, hipStream_t s)
 {
-    if (isU) hipLaunchKernelGGL(k_adi_spec<true>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
-    else hipLaunchKernelGGL(k_adi_spec<false>, dim3(nblocks, 1, b.n), dim3(64), 0, s, b.dv, shift);
+    const dim3 g(nblocks + (bnd ? 1 : 0), 1, b.n);
+    if (isU) hipLaunchKernelGGL(k_adi_spec<true>, g, dim3(64), 0, s, b.dv, shift, bnd);
+    else hipLaunchKernelGGL(k_adi_spec<false>, g, dim3(64), 0, s, b.dv, shift, bnd);
 }
 
 void hd_launch_adi_solve(const HdBatch& b, int isU, int shift, hipStream_t s)
@@ -3093,14 +3277,16 @@ void hd_launch_explicit_margins(const HdBatch& b, int isU, int cells, hipStream_
     else hipLaunchKernelGGL(k_explicit_margins<false>, dim3(nl, 1, b.n), dim3(64), 0, s, b.dv, cells);
 }
 
-void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, hipStream_t s)
+void hd_launch_smooth_u_explicit_v(const HdBatch& b, int bw, int ntiles, int adv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_smooth_u_explicit_v, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_smooth_u_explicit_v, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0,
+                       adv);
 }
 
-void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)
+void hd_launch_smooth_v_explicit_u(const HdBatch& b, int bw, int ntiles, int adv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_smooth_v_explicit_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_smooth_v_explicit_u, row_grid(b, bw, ntiles), dim3(bw), 0, s, b.dv, ntiles > 0 ? 1 : 0,
+                       adv);
 }
 
 void hd_launch_explicit_rows_u(const HdBatch& b, int bw, int ntiles, hipStream_t s)

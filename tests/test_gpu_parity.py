@@ -215,6 +215,57 @@ def test_run_equals_step():
     assert_same_state(g1, g2, "graph replay vs step")
 
 
+def open_basin(types, N=30, M=26, nsteps=40):
+    """A basin wet up to all four edges, with the boundary series and no
+    narrow-channel mode: each edge line's first run is the whole edge, so
+    every corner cell is written by two edges, and the sweeps' runs write the
+    boundary values themselves.  types: boundary type of the edges up
+    (column M), down (column 2), left (row 2), right (row N); 2 is a
+    discharge (with a sediment concentration), anything else a level."""
+    rng = np.random.default_rng(5)
+    hn = 4.0 + rng.normal(0.0, 0.05, (N, M))
+    total_time = 4
+    hours = np.arange(total_time, dtype=np.float64)
+    series = {}
+    for e, (key, cc) in enumerate((("bc_up", "CC_u"), ("bc_down", "CC_d"), ("bc_left", "CC_l"),
+                                   ("bc_right", "CC_r"))):
+        if types[e] == 2:
+            series[key] = 0.2 + 0.05 * hours
+            series[cc] = 0.01 + 0.002 * hours
+        else:
+            series[key] = 0.05 * np.sin(hours + e)
+    header = synth.make_case_header(10.0, 10.0, 2.0, 20, H_TINH=0.02, heso=0.94)
+    case = synth._finish("open_basin", N, M, 20, header, hn, 0.025, 1e-6, types, series, total_time,
+                         nsteps * 2.0)
+    case.sediment_start = 0.0
+    return case
+
+
+@pytest.mark.parametrize("types", [(0, 2, 2, 1), (1, 1, 1, 1), (2, 2, 2, 2)])
+def test_open_edges_and_corners_bitwise(types):
+    """Boundary values written by the sweeps' runs (no k_boundary launch)
+    against the checker's Update_Boundary_Value, at every edge and at corners
+    where a level meets a discharge, two levels or two discharges meet, with
+    the time interpolated within the hour: each half-step, then 30 steps in
+    one run."""
+    case = open_basin(types)
+    o, g = pair(case)
+    op = case.opts()
+    t = 0.0
+    g.profile(True)
+    for which in (0, 1):
+        t = t + case.dT * 0.5
+        o.halfstep(which, t, op)
+        g.halfstep(which, t, op)
+        assert_same_state(o, g, f"half-step {which}")
+    times = g.kernel_times()
+    g.profile(False)
+    assert "boundary" not in times and times["adi_spec_uz"][1] == 1
+    o.run(t, 30, op)
+    g.run(t, 30, op)
+    assert_same_state(o, g, f"open basin {types}, 31 steps")
+
+
 def overflow_case():
     """A channel whose rows are cut by three dry walls into four wet runs,
     with segment_limit = 3."""
