@@ -262,8 +262,7 @@ __device__ __forceinline__ void advance_time(const HdDevG& d)
 
 /* The first wet run of each edge line (column M, column 2, row 2, row N):
  * its first cell a and its cells [lo, hi] clipped to [2, max(M, N)] as
- * k_boundary's loop (hi < lo when the line is dry).  The same for every run
- * of a sweep, so loaded once per workgroup. */
+ * k_boundary's loop (hi < lo when the line is dry). */
 struct EdgeRuns {
     int lo[4], hi[4], a[4];
 };
@@ -345,10 +344,15 @@ __device__ __forceinline__ bool boundary_time(const HdDevG& d, int& t1, double& 
  * rule gives.  The device time is read here and advanced by the next launch
  * (k_smooth_*_explicit_*). */
 template <bool isU>
-__device__ __forceinline__ void run_boundary(const HdDevG& d, const EdgeRuns& er, int line, int first, int last,
-                                             int lane)
+__device__ __forceinline__ void run_boundary(const HdDevG& d, int line, int first, int last, int lane)
 {
     const int N = d.N, M = d.M;
+    /* only runs on an edge line or touching one can own a cell: most runs
+     * leave here without a load */
+    if (!(isU ? (line == M || line == 2 || first == 2 || last == N)
+              : (line == 2 || line == N || first == 2 || last == M)))
+        return;
+    const EdgeRuns er = edge_runs(d);
     /* which cells k of this run each edge writes: [klo, khi] */
     int klo[4], khi[4];
     bool any = false, along = false;
@@ -1288,14 +1292,10 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
     double* tq = isU ? d.t_u : d.t_v;
     const double* hh = d.h;
     const int nbr = (lane + 1) & (kWave - 1);
-    EdgeRuns er{};
-    if (bnd) {
-        er = edge_runs(d);
-        /* the extra last workgroup: the across edges' cells */
-        if ((int)blockIdx.x == (int)gridDim.x - 1) {
-            across_boundary<isU>(d, er, lane);
-            return;
-        }
+    /* the extra last workgroup: the across edges' cells that no run holds */
+    if (bnd && (int)blockIdx.x == (int)gridDim.x - 1) {
+        across_boundary<isU>(d, edge_runs(d), lane);
+        return;
     }
     const RunSlice sl = run_slice(cnt, (int)gridDim.x - bnd);
     for (int ri = sl.start; ri < sl.end; ri += sl.step) {
@@ -1315,7 +1315,7 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
         }
         /* this run's boundary cells, before its end values are read (one
          * wave: its own stores are seen by its later loads) */
-        if (bnd) run_boundary<isU>(d, er, line, first, last, lane);
+        if (bnd) run_boundary<isU>(d, line, first, last, lane);
         const double TZ_l = tz[at(first)];
         const double TZ_r = tz[at(last)];
         const double ub1 = isU ? d.ubt[line] : d.vbd[line];
