@@ -260,6 +260,15 @@ __device__ __forceinline__ void advance_time(const HdDevG& d)
     if (t1 < 0 || t1 + 1 >= d.total_time || t1 + 1 >= d.series_len) d.flags[1] = 1;
 }
 
+/* The repaired-chunk count of the sweep just finished (direction 0 UZ, 1
+ * VZ) becomes the one the next sweep of that direction reads; one thread,
+ * in the launch after the sweep. */
+__device__ __forceinline__ void roll_repairs(const HdDevG& d, int dir)
+{
+    d.run_cnt[2 + dir] = d.flags[4 + dir];
+    d.flags[4 + dir] = 0;
+}
+
 /* The first wet run of each edge line (column M, column 2, row 2, row N):
  * its first cell a and its cells [lo, hi] clipped to [2, max(M, N)] as
  * k_boundary's loop (hi < lo when the line is dry). */
@@ -1267,6 +1276,10 @@ __device__ __forceinline__ Pre shfl_pre(const Pre& a, int src)
  * depths, edge values and the first cells' preindex inputs are loaded
  * together, so a short run costs one memory round trip after its table
  * entry. */
+/* repaired chunks of a sweep above which the next sweep of the same
+ * direction takes 8 rows per lane */
+constexpr int kAdaptRuns = 32;
+
 template <bool isU>
 __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv, int shift, int bnd)
 {
@@ -1292,6 +1305,12 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
     double* tq = isU ? d.t_u : d.t_v;
     const double* hh = d.h;
     const int nbr = (lane + 1) & (kWave - 1);
+    /* rows per lane of the speculative forward sweep: 8 while the previous
+     * sweep of this direction repaired many runs (a flow starting up), else
+     * as few as the run needs; this sweep's repaired chunks are counted for
+     * the next one (moved into place by the explicit pass that follows) */
+    const int minL = d.run_cnt[isU ? 2 : 3] > kAdaptRuns ? kSpecMaxL : 1;
+    int* fails = d.flags + (isU ? 4 : 5);
     /* the extra last workgroup: the across edges' cells that no run holds */
     if (bnd && (int)blockIdx.x == (int)gridDim.x - 1) {
         across_boundary<isU>(d, edge_runs(d), lane);
@@ -1392,7 +1411,7 @@ __global__ void __launch_bounds__(64, 2) k_adi_spec(const HdDev* __restrict__ dv
                     g0 = 1;
                 }
                 const int g1 = (re < sn - 1 ? re : sn - 1) - rb;
-                spec_forward<true>(sA, sB, sC, sD, g0, g1, ap, bp);
+                spec_forward<true>(sA, sB, sC, sD, g0, g1, ap, bp, minL, fails);
                 if (c == 0 && sn > 0) {
                     __syncthreads();
                     if (lane == 0) { sC[0] = -sC[0] / sB[0]; sD[0] = sD[0] / sB[0]; }
@@ -2118,7 +2137,10 @@ __device__ __forceinline__ void dry_store(const HdDevG& d, int i, int j, int hor
 __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __restrict__ dv, int tiled, int adv)
 {
     const HdDevG& d = dev_view(dv);
-    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) advance_time(d);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (adv) advance_time(d);
+        roll_repairs(d, 0);
+    }
     int i, j;
     if (!row_cell(d, tiled, i, j)) return;
     const int N = d.N, M = d.M, W = d.W;
@@ -2169,7 +2191,10 @@ __global__ void __launch_bounds__(256) k_smooth_u_explicit_v(const HdDev* __rest
 __global__ void __launch_bounds__(256) k_smooth_v_explicit_u(const HdDev* __restrict__ dv, int tiled, int adv)
 {
     const HdDevG& d = dev_view(dv);
-    if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) advance_time(d);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (adv) advance_time(d);
+        roll_repairs(d, 1);
+    }
     int i, j;
     if (!row_cell(d, tiled, i, j)) return;
     const int N = d.N, M = d.M, W = d.W;
@@ -3018,7 +3043,11 @@ __global__ void __launch_bounds__(256) k_uvz_segments(const HdDev* __restrict__ 
     const HdDevG& d = dev_view(dv);
     const int nseg = nrowb + ncolb + nfs;
     if ((int)blockIdx.x < nseg) {
-        find_limits_block(d, blockIdx.x, nrowb, ncolb, nfsr, count_wet, d.dryflag, kd, kn);
+        /* the column scans, the launch's longest workgroups, are dispatched
+         * first: workgroup b < ncolb takes column block b */
+        const int b = blockIdx.x;
+        const int bid = b < ncolb ? nrowb + b : (b < ncolb + nrowb ? b - ncolb : b);
+        find_limits_block(d, bid, nrowb, ncolb, nfsr, count_wet, d.dryflag, kd, kn);
         return;
     }
     const int t = ((int)blockIdx.x - nseg) * 4 + (threadIdx.x >> 6);

@@ -186,7 +186,7 @@ __device__ __forceinline__ double bp_known_e(double A, double D, double bp, doub
  * uniform control; ends with a barrier. */
 template <bool kFastRepair>
 __device__ __forceinline__ void spec_forward(const double* sA, double* sB, double* sC, double* sD, int g0,
-                                             int g1, double& ap, double& bp)
+                                             int g1, double& ap, double& bp, int minL = 1, int* fails = nullptr)
 {
     const int lane = threadIdx.x;
     const int n = g1 - g0 + 1;
@@ -207,7 +207,11 @@ __device__ __forceinline__ void spec_forward(const double* sA, double* sB, doubl
         __syncthreads();
         return;
     }
-    const int L = (n + kSpecWave - 1) / kSpecWave; /* rows per lane, <= kSpecMaxL */
+    /* rows per lane, <= kSpecMaxL: at least minL (more rows per lane, fewer
+     * lanes, a shorter chain of lane-by-lane repairs when speculation fails;
+     * the result is the same for any L) */
+    const int L0 = (n + kSpecWave - 1) / kSpecWave;
+    const int L = L0 < minL ? minL : L0;
     const int sk = g0 + lane * L;
     const int ek = sk + L <= g1 + 1 ? sk + L : g1 + 1;
     const bool act = sk <= g1;
@@ -257,6 +261,7 @@ __device__ __forceinline__ void spec_forward(const double* sA, double* sB, doubl
         double pa = __shfl_up(ca, 1, kSpecWave), pb = __shfl_up(cb, 1, kSpecWave);
         bool ok = exact || !act || (same_bits(wa, pa) && same_bits(wb, pb));
         unsigned long long bad = __ballot(!ok);
+        if (bad && fails && lane == 0) atomicAdd(fails, 1);
         if constexpr (kFastRepair) {
             /* Where the flow starts from rest every failure has the lane's
              * ap bits right (ap depends on A, B, C only and merges; bp decays
