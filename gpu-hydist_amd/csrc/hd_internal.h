@@ -33,8 +33,10 @@
  * hmax[4]; tmp_u / tmp_v, the smoothing ping-pong partners of t_u / t_v;
  * AA..Bp line-major rows and factors; seg_end / seg_sn per-run side tables;
  * runs_u / runs_v packed wet runs (line, first, last, 0) of the UZ / VZ
- * sweeps and run_cnt their entry counts; flags [0] segment overflow, [1]
- * boundary range; wet_count [0] run accumulator, [1] one-shot count; tdev
+ * sweeps and run_cnt [0] / [1] their entry counts, [2] / [3] the chunks the
+ * last UZ / VZ sweep repaired; flags [0] segment overflow, [1] boundary
+ * range, [4] / [5] the chunks the running UZ / VZ sweep repairs; wet_count
+ * [0] run accumulator, [1] one-shot count; tdev
  * the device time; tiles / ntiles the active 64-column row tiles; wetb the
  * wet bits of the segment finder, one 64-bit word per 64-cell tile row
  * ((N+3) x ceil((M+3)/64) words, bit b of word (i, w) for cell
