@@ -405,8 +405,8 @@ def river_line(a):
 
 
 def sediment_line(a):
-    """The river with suspended sediment and bed change every step (the
-    config-3 physics from t = 0)."""
+    """The 980 x 1048 river with suspended sediment and bed change every
+    step, both from t = 0."""
     from hydist_amd import synth
     from hydist_amd.case import BED_EVERY_STEP
     case = synth.config(3)
@@ -473,8 +473,9 @@ def cnn_line(a):
 
 
 def surrogate_line(a):
-    """Config 5 in a bounded form: train the surrogate on (state, state + 60
-    steps) pairs from 8 inflow variants of the config-3 river, then run a day
+    """The surrogate-accelerated bed change in a bounded form: train the
+    surrogate on (state, state + 60 steps) pairs from 8 inflow variants of
+    the 980 x 1048 sediment river, then run a day
     of bed change every step three ways from the same state at hour 2 --
     plain FP64 ADI, the surrogate with an FP64 check after every call
     (sediment every step, and with morphological factor 6), and an FP64 twin

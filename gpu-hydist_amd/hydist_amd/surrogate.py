@@ -70,8 +70,9 @@ SCALES = dict(s_u=1.0, s_v=1.0, s_z=1.0, s_b=1e30, s_q=3000.0, s_h=10.0, d_u=0.0
 
 
 def river_variant(k: int, scale: float = 1.0, hours: float = 30.0, seed: int = 50):
-    """Config-3 river (sediment, bed change every step from hour 2) with a
-    seeded inflow hydrograph: mean, amplitude and period drawn per variant."""
+    """The 980 x 1048 sediment river (bed change every step from hour 2)
+    with a seeded inflow hydrograph: mean, amplitude and period drawn per
+    variant."""
     rng = np.random.default_rng(seed + k)
     qm = 3000.0 * scale * float(rng.uniform(0.75, 1.25))
     qa = qm * float(rng.uniform(0.15, 0.4))
@@ -330,8 +331,8 @@ def evaluate(scale: float = 1.0, variants: int = 8, pairs: int = 250, K: int = 6
              z_bound: float = 5e-3, z_frac: float = 1e-3, mass_bound: float = 1e-4, log_checks: bool = False,
              extra_z_fracs=(), perfect_flow: bool = False, log=print) -> dict:
     """Train the surrogate on plain-ADI pairs and compare the accelerated
-    bed-change run with the plain one on the config-3 river (bed change every
-    step, from hour 2): simulated hours per wall second of both, and the bed
+    bed-change run with the plain one on the 980 x 1048 sediment river (bed
+    change every step, from hour 2): simulated hours per wall second of both, and the bed
     (dH) difference after one day and after `days` days.  Beside them, the
     FP64 method against itself: a twin run whose inflow series is scaled by
     1 + 1e-12, the smallest difference any other method could have, and the

@@ -2,12 +2,12 @@
 checker at hourly checkpoints (the reference's own output interval,
 Engine.py:398-420):
 
-* config 1, the 300 x 60 channel, all 3600 steps;
-* config 2, the 980 x 1048 river, all 4320 steps (6 simulated hours);
-* config 3, the river with suspended sediment from hour 1 and bed change
+* the 300 x 60 channel, all 3600 steps;
+* the 980 x 1048 river, all 4320 steps (6 simulated hours);
+* the 980 x 1048 river with suspended sediment from hour 1 and bed change
   from hour 2, through hour 3, with the bed change once per period as
   Engine.py:364 triggers it and every step;
-* config 4, four members of the seeded ensemble draw (sides 256-1024),
+* four members of the seeded 64-river ensemble draw (sides 256-1024),
   stepped together as one ensemble with sediment and bed change, each
   member against its own checker.
 """
@@ -52,20 +52,20 @@ def run_compare(case, chunks, label):
     return t
 
 
-def test_channel_config1_full_window():
+def test_channel_300x60_full_window():
     case = synth.config(1)
     t = run_compare(case, [900] * 4, "channel 300x60")
     assert t == 3600 * case.dT
 
 
-def test_river_config2_full_window():
+def test_river_980x1048_full_window():
     case = synth.config(2)
     t = run_compare(case, [720] * 6, "river 980x1048")
     assert t == 6 * 3600.0
 
 
 @pytest.mark.parametrize("bed_mode", [BED_PERIODIC, BED_EVERY_STEP])
-def test_river_sediment_config3_from_hour1_and_hour2(bed_mode):
+def test_sediment_river_from_hour1_and_hour2(bed_mode):
     case = synth.config(3)
     case.sediment_start = 3600.0
     case.bed_change_start = 7200.0
